@@ -1,0 +1,359 @@
+// BVH construction for gfx950: leaf (swept) boxes + Morton codes, a stable LDS radix
+// sort of (code, triangle id), the Karras radix tree and an atomic bottom-up refit.
+// Output is the reference wire format (src/build_bvh.cu:309-378):
+//   bboxes [B, 2n-1, 6] (min xyz, max xyz), tree [B, 2n-1, 4] int64 (parent, left, right,
+//   id), internal rows 0..n-2 (row 0 = root), leaf rows n-1..2n-2 in sorted order.
+#include <float.h>
+
+#include "ccc_geom.h"
+#include "ccc_internal.h"
+
+namespace ccc {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+
+template <typename T> struct Lim;
+template <> struct Lim<double> { static constexpr double max() { return DBL_MAX; } };
+template <> struct Lim<float> { static constexpr float max() { return FLT_MAX; } };
+
+// Spread the low 10 bits of v to every third bit (bit k -> bit 3k).
+__device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// bvh.cu:54-62: each normalised coordinate is scaled by 1024, clamped to [0, 1023] with
+// fmin/fmax (NaN -> 0) and truncated; code = x-bits * 4 + y-bits * 2 + z-bits.
+__device__ __forceinline__ uint32_t morton30(double x, double y, double z) {
+    x = fmin(fmax(x * 1024.0, 0.0), 1023.0);
+    y = fmin(fmax(y * 1024.0, 0.0), 1023.0);
+    z = fmin(fmax(z * 1024.0, 0.0), 1023.0);
+    return spread_bits10((uint32_t)x) * 4u + spread_bits10((uint32_t)y) * 2u + spread_bits10((uint32_t)z);
+}
+__device__ __forceinline__ uint32_t morton30(float x, float y, float z) {
+    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    y = fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+    z = fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+    return spread_bits10((uint32_t)x) * 4u + spread_bits10((uint32_t)y) * 2u + spread_bits10((uint32_t)z);
+}
+
+// Leaf boxes (build_bvh.cu:212-219, :452-460) and Morton codes of the centroids
+// (build_bvh.cu:26-71).  The scene box is the union of all leaf boxes with the reduction's
+// initial value AABB<T>() = [-MAX, +MAX] (build_bvh.cu:243-244, :505-506; aabb.cu:25-33);
+// min/max against +-MAX leave that initial value unchanged whatever the boxes are, so the
+// scene box is [-MAX, +MAX] and is used here directly.
+template <typename T, bool MOTION>
+__global__ void k_leaf_boxes(const T *__restrict__ t0, const T *__restrict__ t1, int64_t total,
+                             int64_t n, T *__restrict__ boxes, uint32_t *__restrict__ codes,
+                             int32_t *__restrict__ ids) {
+    int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total) return;
+    const T *a = t0 + 9 * gi;
+    V3<T> a0 = ld3(a), a1 = ld3(a + 3), a2 = ld3(a + 6);
+    Box<T> bx = tri_box(a0, a1, a2);
+    V3<T> c = mk((a0.x + a1.x + a2.x) / (T)3.0, (a0.y + a1.y + a2.y) / (T)3.0, (a0.z + a1.z + a2.z) / (T)3.0);
+    if (MOTION) {
+        const T *b = t1 + 9 * gi;
+        V3<T> b0 = ld3(b), b1 = ld3(b + 3), b2 = ld3(b + 6);
+        bx = unite(bx, tri_box(b0, b1, b2));
+        V3<T> c2 = mk((b0.x + b1.x + b2.x) / (T)3.0, (b0.y + b1.y + b2.y) / (T)3.0, (b0.z + b1.z + b2.z) / (T)3.0);
+        c = mk((c.x + c2.x) / (T)2.0, (c.y + c2.y) / (T)2.0, (c.z + c2.z) / (T)2.0);
+    }
+    T *o = boxes + 6 * gi;
+    o[0] = bx.mn.x; o[1] = bx.mn.y; o[2] = bx.mn.z; o[3] = bx.mx.x; o[4] = bx.mx.y; o[5] = bx.mx.z;
+    const T smin = -Lim<T>::max(), smax = Lim<T>::max();
+    T x = (c.x - smin) / (smax - smin);
+    T y = (c.y - smin) / (smax - smin);
+    T z = (c.z - smin) / (smax - smin);
+    codes[gi] = morton30(x, y, z);
+    ids[gi] = (int32_t)(gi % n);
+}
+
+// ---- stable LSD radix sort of (code, id), 8 bits per pass --------------------------------
+// Pass = per-tile digit histogram -> per-element exclusive scan (digit-major) -> stable
+// scatter.  The scatter ranks keys inside a wave with a ballot multisplit (8 ballots give
+// the lanes holding the same digit) and across the block's four waves with LDS counters.
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+                                                            uint32_t *__restrict__ hist, int nb) {
+    __shared__ uint32_t h[256];
+    const int b = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const uint32_t *k = keys + (int64_t)b * n;
+    for (int r = 0; r < kSortRounds; ++r) {
+        int64_t i = (int64_t)blk * kSortTile + r * kSortThreads + tid;
+        if (i < n) atomicAdd(&h[(k[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[((int64_t)b * 256 + tid) * nb + blk] = h[tid];
+}
+
+// Exclusive scan of one element's [256][nb] histogram, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *__restrict__ hist, int nb) {
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    uint32_t *h = hist + (int64_t)blockIdx.x * 256 * nb;
+    const int64_t total = (int64_t)256 * nb;
+    const int64_t chunk = (total + 1023) / 1024;
+    const int64_t lo = tid * chunk, hi = lo + chunk < total ? lo + chunk : total;
+    uint32_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += h[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - s;
+    for (int64_t i = lo; i < hi; ++i) {
+        uint32_t v = h[i];
+        h[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
+                                                               uint32_t *__restrict__ kout, int32_t *__restrict__ vout,
+                                                               int64_t n, int shift, const uint32_t *__restrict__ hist, int nb) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t wcnt[kSortThreads / 64][256];
+    const int b = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x, w = tid >> 6;
+    const int64_t off = (int64_t)b * n;
+    gbase[tid] = hist[((int64_t)b * 256 + tid) * nb + blk];
+    run[tid] = 0;
+    for (int r = 0; r < kSortRounds; ++r) {
+#pragma unroll
+        for (int q = 0; q < kSortThreads / 64; ++q) wcnt[q][tid] = 0;
+        __syncthreads();
+        const int64_t i = (int64_t)blk * kSortTile + r * kSortThreads + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[off + i] : 0u;
+        const int32_t val = valid ? vin[off + i] : 0;
+        const uint32_t d = (key >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            unsigned long long bb = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const unsigned rank = lanes_below(peers);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t base = run[tid], acc = 0;
+#pragma unroll
+            for (int q = 0; q < kSortThreads / 64; ++q) {
+                uint32_t c = wcnt[q][tid];
+                wcnt[q][tid] = base + acc;
+                acc += c;
+            }
+            run[tid] = base + acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const int64_t dst = (int64_t)gbase[d] + wcnt[w][d] + rank;
+            kout[off + dst] = key;
+            vout[off + dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- Karras radix tree ----------------------------------------------------------------------
+// bvh.cu:70-89 LongestCommonPrefix: -1 outside [0, n-1]; equal codes extend the prefix with
+// the ids' common prefix (32 + clz(id_i ^ id_j)).
+__device__ __forceinline__ int lcp_delta(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j > n - 1) return -1;
+    const uint32_t a = codes[i], c = codes[j];
+    if (a == c) return 32 + __clz((int)((uint32_t)ids[i] ^ (uint32_t)ids[j]));
+    return __clz((int)(a ^ c));
+}
+
+// Karras 2012, Algorithm 1 (what build_bvh.cu:74-143 computes): node i finds its range
+// direction, the far end j by exponential + binary search and the split by binary search.
+// Writes left/right/self of row i and the parent field of both children's rows.
+__global__ void k_karras(const uint32_t *__restrict__ codes_all, const int32_t *__restrict__ ids_all, int64_t n,
+                         int64_t *__restrict__ tree_all) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint32_t *codes = codes_all + (int64_t)b * n;
+    const int32_t *ids = ids_all + (int64_t)b * n;
+    int64_t *tree = tree_all + (int64_t)b * (2 * n - 1) * 4;
+    const int d = (lcp_delta(codes, ids, n, i, i + 1) - lcp_delta(codes, ids, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = lcp_delta(codes, ids, n, i, i - d);
+    int64_t lmax = 2;
+    while (lcp_delta(codes, ids, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (lcp_delta(codes, ids, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = lcp_delta(codes, ids, n, i, j);
+    int64_t s = 0;
+    for (int64_t div = 2;; div *= 2) {
+        const int64_t t = (l + div - 1) / div;
+        if (lcp_delta(codes, ids, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    const int64_t left = (lo == g) ? (n - 1 + g) : g;
+    const int64_t right = (hi == g + 1) ? (n - 1 + g + 1) : (g + 1);
+    tree[4 * i + 1] = left;
+    tree[4 * i + 2] = right;
+    tree[4 * i + 3] = -1;
+    if (i == 0) tree[0] = -1;
+    tree[4 * left + 0] = i;
+    tree[4 * right + 0] = i;
+}
+
+__device__ __forceinline__ double ld_coherent(const double *p) {
+    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ float ld_coherent(const float *p) {
+    unsigned v = __hip_atomic_load((const unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float(v);
+}
+__device__ __forceinline__ int64_t ld_coherent_i64(const int64_t *p) {
+    return (int64_t)__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Leaf rows and bottom-up refit (build_bvh.cu:145-209): leaf at sorted position p gets the
+// box of triangle ids[p]; the second thread to reach an internal node writes the union of
+// its children and climbs on.  A climb is bounded by n steps.
+template <typename T>
+__global__ void k_refit(const T *__restrict__ boxes_all, const int32_t *__restrict__ ids_all, int64_t n,
+                        int *__restrict__ counters_all, T *__restrict__ bboxes_all, int64_t *__restrict__ tree_all,
+                        unsigned *__restrict__ err) {
+    const int b = blockIdx.y;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int64_t rows = 2 * n - 1;
+    const T *boxes = boxes_all + (int64_t)b * n * 6;
+    const int32_t *ids = ids_all + (int64_t)b * n;
+    int *counters = counters_all + (int64_t)b * (n > 1 ? n - 1 : 1);
+    T *bb = bboxes_all + (int64_t)b * rows * 6;
+    int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    const int64_t row = n - 1 + p;
+    const int32_t id = ids[p];
+    const T *src = boxes + 6 * (int64_t)id;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bb[6 * row + k] = src[k];
+    tree[4 * row + 1] = -1;
+    tree[4 * row + 2] = -1;
+    tree[4 * row + 3] = id;
+    if (n == 1) {
+        tree[4 * row + 0] = -1;
+        return;
+    }
+    int64_t cur = tree[4 * row + 0];
+    for (int64_t step = 0; cur >= 0; ++step) {
+        if (step > n) { atomicOr(err, DEV_BAD_TREE); break; }
+        __threadfence();
+        const int old = __hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) break;
+        __threadfence();
+        const int64_t l = tree[4 * cur + 1], r = tree[4 * cur + 2];
+        Box<T> bl, br;
+        bl.mn = mk(ld_coherent(bb + 6 * l + 0), ld_coherent(bb + 6 * l + 1), ld_coherent(bb + 6 * l + 2));
+        bl.mx = mk(ld_coherent(bb + 6 * l + 3), ld_coherent(bb + 6 * l + 4), ld_coherent(bb + 6 * l + 5));
+        br.mn = mk(ld_coherent(bb + 6 * r + 0), ld_coherent(bb + 6 * r + 1), ld_coherent(bb + 6 * r + 2));
+        br.mx = mk(ld_coherent(bb + 6 * r + 3), ld_coherent(bb + 6 * r + 4), ld_coherent(bb + 6 * r + 5));
+        Box<T> u = unite(bl, br);
+        T *o = bb + 6 * cur;
+        o[0] = u.mn.x; o[1] = u.mn.y; o[2] = u.mn.z; o[3] = u.mx.x; o[4] = u.mx.y; o[5] = u.mx.z;
+        cur = tree[4 * cur + 0];
+    }
+}
+
+template <typename T>
+static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, T *bboxes, int64_t *tree,
+                        hipStream_t s) {
+    Workspace ws(ctx, s);
+    const int64_t total = B * n;
+    const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    T *boxes = ws.get_as<T>(total * 6);
+    uint32_t *ka = ws.get_as<uint32_t>(total), *kb = ws.get_as<uint32_t>(total);
+    int32_t *va = ws.get_as<int32_t>(total), *vb = ws.get_as<int32_t>(total);
+    uint32_t *hist = ws.get_as<uint32_t>((size_t)B * 256 * nb);
+    int *counters = ws.get_as<int>((size_t)B * (n > 1 ? n - 1 : 1));
+    unsigned *err = ws.get_as<unsigned>(1);
+    if (ws.failed()) return CCC_ERR_NO_MEMORY;
+    ccc_status st;
+    if ((st = fill_u32(counters, B * (n > 1 ? n - 1 : 1), 0u, s)) != CCC_OK) return st;
+    if ((st = fill_u32(err, 1, 0u, s)) != CCC_OK) return st;
+
+    if (t1) hipLaunchKernelGGL((k_leaf_boxes<T, true>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
+    else hipLaunchKernelGGL((k_leaf_boxes<T, false>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
+    CCC_LAUNCH_CHECK("k_leaf_boxes");
+
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nb, B), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_sort_scan, dim3(B), dim3(1024), 0, s, hist, nb);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nb, B), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, hist, nb);
+        CCC_LAUNCH_CHECK("radix sort");
+        uint32_t *tk = ka; ka = kb; kb = tk;
+        int32_t *tv = va; va = vb; vb = tv;
+    }
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree);
+        CCC_LAUNCH_CHECK("k_karras");
+    }
+    hipLaunchKernelGGL((k_refit<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, boxes, va, n, counters, bboxes, tree, err);
+    CCC_LAUNCH_CHECK("k_refit");
+    unsigned herr = 0;
+    CCC_HIP_TRY(hipMemcpyAsync(&herr, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    CCC_HIP_TRY(hipStreamSynchronize(s));
+    if (herr) {
+        set_error("BVH refit did not terminate (corrupt tree)");
+        return CCC_ERR_BAD_TREE;
+    }
+    return CCC_OK;
+}
+
+static ccc_status build_entry(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
+                              void *bboxes, int64_t *tree, void *stream) {
+    clear_error();
+    if (!ctx) return bad_arg("context is NULL");
+    if (B < 0 || n < 1) return bad_arg("need batch >= 0 and n >= 1 triangles");
+    if (n > (int64_t)INT32_MAX / 2) return bad_arg("too many triangles");
+    if (B == 0) return CCC_OK;
+    if (!t0 || !bboxes || !tree) return bad_arg("NULL tensor pointer");
+    CCC_HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == CCC_FLOAT64) return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, (double *)bboxes, tree, s);
+    if (dtype == CCC_FLOAT32) return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, (float *)bboxes, tree, s);
+    return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+}
+
+}  // namespace ccc
+
+extern "C" {
+
+ccc_status ccc_bvh(ccc_context *ctx, const void *triangles, int64_t batch, int64_t n, int32_t dtype, void *bboxes,
+                   int64_t *tree, void *stream) {
+    return ccc::build_entry(ctx, triangles, nullptr, batch, n, dtype, bboxes, tree, stream);
+}
+
+ccc_status ccc_bvh_motion(ccc_context *ctx, const void *triangles, const void *triangles_next, int64_t batch, int64_t n,
+                          int32_t dtype, void *bboxes, int64_t *tree, void *stream) {
+    if (!triangles_next) {
+        ccc::set_error("triangles_next is NULL");
+        return CCC_ERR_BAD_ARG;
+    }
+    return ccc::build_entry(ctx, triangles, triangles_next, batch, n, dtype, bboxes, tree, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,532 @@
+// Device geometry for the collision hot path (gfx950).
+//
+// Every expression keeps the operation order of the reference (src/figures.cu,
+// src/utils.cu, include/double_vec_ops.h, src/collisions_*.cu); the library is compiled
+// with -ffp-contract=off and contractions are written out as fma() only where the
+// reference's compiler contracts a `coef += x*y*z` accumulation (figures.cu:150-281).
+// CUDA's min/max on floating point are fmin/fmax (NaN-ignoring) and are written so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccc {
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> struct Box { V3<T> mn, mx; };
+
+__device__ __forceinline__ double tmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float tmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float tmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double tabs(double a) { return fabs(a); }
+__device__ __forceinline__ float tabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float tsqrt(float a) { return sqrtf(a); }
+
+template <typename T> __device__ __forceinline__ V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T> __device__ __forceinline__ V3<T> operator*(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <typename T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ T norm2(V3<T> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+// utils.cu:91-93 normalize(a) = a * rsqrt(|a|^2), evaluated as a * (1 / sqrt(|a|^2)).
+template <typename T> __device__ __forceinline__ V3<T> normalize(V3<T> a) {
+    T n2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    return a * ((T)1 / tsqrt(n2));
+}
+// utils.cu:59-62 vec_sq_diff: the squared distance is returned as float.
+template <typename T> __device__ __forceinline__ float sq_diff(V3<T> a, V3<T> b) {
+    V3<T> d = a - b;
+    return (float)dot(d, d);
+}
+template <typename T> __device__ __forceinline__ bool veq(V3<T> a, V3<T> b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+template <typename T> __device__ __forceinline__ V3<T> ld3(const T *p) { return mk(p[0], p[1], p[2]); }
+
+// aabb.cu:51-56 union
+template <typename T> __device__ __forceinline__ Box<T> unite(Box<T> a, Box<T> b) {
+    return Box<T>{mk(tmin(a.mn.x, b.mn.x), tmin(a.mn.y, b.mn.y), tmin(a.mn.z, b.mn.z)),
+                  mk(tmax(a.mx.x, b.mx.x), tmax(a.mx.y, b.mx.y), tmax(a.mx.z, b.mx.z))};
+}
+// aabb.cu:101-111 checkOverlap(bbox1, bbox2, threshold)
+template <typename T> __device__ __forceinline__ bool overlap(const Box<T> &b1, const Box<T> &b2, T thr) {
+    return (b1.mn.x <= (b2.mx.x + thr)) && (b1.mx.x >= (b2.mn.x - thr)) &&
+           (b1.mn.y <= (b2.mx.y + thr)) && (b1.mx.y >= (b2.mn.y - thr)) &&
+           (b1.mn.z <= (b2.mx.z + thr)) && (b1.mx.z >= (b2.mn.z - thr));
+}
+template <typename T> __device__ __forceinline__ Box<T> ld_box(const T *p) {
+    return Box<T>{mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5])};
+}
+template <typename T> __device__ __forceinline__ Box<T> point_box(V3<T> v) { return Box<T>{v, v}; }
+// figures.cu:37-41 Triangle::ComputeBBox
+template <typename T> __device__ __forceinline__ Box<T> tri_box(V3<T> a, V3<T> b, V3<T> c) {
+    return Box<T>{mk(tmin(a.x, tmin(b.x, c.x)), tmin(a.y, tmin(b.y, c.y)), tmin(a.z, tmin(b.z, c.z))),
+                  mk(tmax(a.x, tmax(b.x, c.x)), tmax(a.y, tmax(b.y, c.y)), tmax(a.z, tmax(b.z, c.z)))};
+}
+// figures.cu:346-351 swept box of one moving vertex (v, v + dv)
+template <typename T> __device__ __forceinline__ Box<T> swept_vbox(V3<T> v, V3<T> dv) {
+    V3<T> w = v + dv;
+    return Box<T>{mk(tmin(v.x, w.x), tmin(v.y, w.y), tmin(v.z, w.z)), mk(tmax(v.x, w.x), tmax(v.y, w.y), tmax(v.z, w.z))};
+}
+
+// A triangle and its motion over the step: figures.cu:295-314 (dv = v(t1) - v(t0)).
+template <typename T> struct MTri { V3<T> v[3]; V3<T> dv[3]; };
+
+template <typename T> __device__ __forceinline__ MTri<T> ld_mtri(const T *t0, const T *t1, int64_t i) {
+    MTri<T> m;
+    const T *a = t0 + 9 * i, *b = t1 + 9 * i;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        m.v[k] = ld3(a + 3 * k);
+        m.dv[k] = ld3(b + 3 * k) - m.v[k];
+    }
+    return m;
+}
+
+// figures.cu:527-544 shareVertex: exact coincidence of any t0 vertices.
+template <typename T> __device__ __forceinline__ bool share_vertex(const V3<T> *a, const V3<T> *b) {
+    bool s = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s = s || veq(a[i], b[j]);
+    return s;
+}
+// figures.cu:547-576 countSharedVerts
+template <typename T> __device__ __forceinline__ int count_shared(const V3<T> *a, const V3<T> *b) {
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c += (veq(a[i], b[0]) || veq(a[i], b[1]) || veq(a[i], b[2])) ? 1 : 0;
+    return c;
+}
+
+// The 15 sub-primitive types (collisions_continuous.cu:46-122 / :250-311,
+// collisions_impulses.cu:30-121 / :691-932).  0-8: edge E[t/3] of x vs edge E[t%3] of y
+// with E = {(0,1),(0,2),(1,2)}; 9-11: face of x vs vertex t-9 of y; 12-14: vertex t-12 of
+// x vs face of y.  corner(t, k) -> (triangle: 0 = x / 1 = y, corner index).
+__device__ __forceinline__ int edge_first(int e) { return e == 2 ? 1 : 0; }
+__device__ __forceinline__ int edge_second(int e) { return e == 0 ? 1 : 2; }
+__device__ __forceinline__ void type_corner(int type, int k, int &side, int &idx) {
+    if (type < 9) {
+        int e = k < 2 ? type / 3 : type % 3;
+        side = k < 2 ? 0 : 1;
+        idx = (k & 1) ? edge_second(e) : edge_first(e);
+    } else if (type < 12) {
+        side = k < 3 ? 0 : 1;
+        idx = k < 3 ? k : type - 9;
+    } else {
+        side = k < 1 ? 0 : 1;
+        idx = k < 1 ? type - 12 : k - 1;
+    }
+}
+
+// Swept box of a sub-primitive of a moving triangle (figures.cu:346-364):
+// kind 0: vertex i; kind 1: edge e; kind 2: whole triangle.
+template <typename T> __device__ __forceinline__ Box<T> mtri_edge_box(const MTri<T> &t, int e) {
+    return unite(swept_vbox(t.v[edge_first(e)], t.dv[edge_first(e)]), swept_vbox(t.v[edge_second(e)], t.dv[edge_second(e)]));
+}
+template <typename T> __device__ __forceinline__ Box<T> mtri_box(const MTri<T> &t) {
+    return unite(unite(swept_vbox(t.v[0], t.dv[0]), swept_vbox(t.v[1], t.dv[1])), swept_vbox(t.v[2], t.dv[2]));
+}
+// Static sub-primitive boxes (figures.cu:58-67; proximity_static.cu:11-98).
+template <typename T> __device__ __forceinline__ Box<T> tri_edge_box(const V3<T> *v, int e) {
+    return unite(point_box(v[edge_first(e)]), point_box(v[edge_second(e)]));
+}
+
+// 15-bit mask of sub-primitive pairs whose swept boxes overlap
+// (collisions_continuous.cu:234-318, threshold 0).
+template <typename T> __device__ __forceinline__ uint32_t moving_mask(const MTri<T> &a, const MTri<T> &b) {
+    Box<T> ea[3], eb[3], va[3], vb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ea[i] = mtri_edge_box(a, i);
+        eb[i] = mtri_edge_box(b, i);
+        va[i] = swept_vbox(a.v[i], a.dv[i]);
+        vb[i] = swept_vbox(b.v[i], b.dv[i]);
+    }
+    Box<T> fa = mtri_box(a), fb = mtri_box(b);
+    uint32_t m = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) m |= overlap(ea[t / 3], eb[t % 3], (T)0) ? (1u << t) : 0u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        m |= overlap(fa, vb[k], (T)0) ? (1u << (9 + k)) : 0u;
+        m |= overlap(va[k], fb, (T)0) ? (1u << (12 + k)) : 0u;
+    }
+    return m;
+}
+// Same filter for static triangles with a threshold (proximity_static.cu:11-98).
+template <typename T> __device__ __forceinline__ uint32_t static_mask(const V3<T> *a, const V3<T> *b, T thr) {
+    Box<T> ea[3], eb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ea[i] = tri_edge_box(a, i); eb[i] = tri_edge_box(b, i); }
+    Box<T> fa = tri_box(a[0], a[1], a[2]), fb = tri_box(b[0], b[1], b[2]);
+    uint32_t m = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) m |= overlap(ea[t / 3], eb[t % 3], thr) ? (1u << t) : 0u;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        m |= overlap(fa, point_box(b[k]), thr) ? (1u << (9 + k)) : 0u;
+        m |= overlap(point_box(a[k]), fb, thr) ? (1u << (12 + k)) : 0u;
+    }
+    return m;
+}
+
+// Moving tetrahedron (figures.cu:74-96) of one (x, y, type) pair, gathered from the soups.
+template <typename T> struct MTet { V3<T> v[4]; V3<T> dv[4]; };
+
+template <typename T>
+__device__ __forceinline__ MTet<T> gather_mtet(const T *t0, const T *t1, int64_t x, int64_t y, int type) {
+    MTet<T> m;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int side, idx;
+        type_corner(type, k, side, idx);
+        int64_t off = 9 * (side ? y : x) + 3 * idx;
+        m.v[k] = ld3(t0 + off);
+        m.dv[k] = ld3(t1 + off) - m.v[k];
+    }
+    return m;
+}
+
+// figures.cu:122-293 coefficients c3..c0 of det[(v_i - v3) + t (dv_i - dv3)], i = 0..2,
+// accumulated in the reference's term order; `coef += p*q*r` is fma(p*q, r, coef).
+template <typename T> __device__ __forceinline__ void cubic_coeffs(const MTet<T> &t, T *c) {
+    const V3<T> v0 = t.v[0] - t.v[3], v1 = t.v[1] - t.v[3], v2 = t.v[2] - t.v[3];
+    const V3<T> dv0 = t.dv[0] - t.dv[3], dv1 = t.dv[1] - t.dv[3], dv2 = t.dv[2] - t.dv[3];
+#define CCC_ACC(acc, p, q, r) acc = tfma((p) * (q), (r), acc)
+    T c3 = 0, c2 = 0, c1 = 0, c0 = 0;
+    CCC_ACC(c3, dv2.z, dv1.y, dv0.x);
+    CCC_ACC(c3, -dv1.z, dv2.y, dv0.x);
+    CCC_ACC(c3, -dv2.z, dv0.y, dv1.x);
+    CCC_ACC(c3, dv0.z, dv2.y, dv1.x);
+    CCC_ACC(c3, dv1.z, dv0.y, dv2.x);
+    CCC_ACC(c3, -dv0.z, dv1.y, dv2.x);
+
+    CCC_ACC(c2, v2.z, dv1.y, dv0.x);
+    CCC_ACC(c2, dv2.z, v1.y, dv0.x);
+    CCC_ACC(c2, dv2.z, dv1.y, v0.x);
+    CCC_ACC(c2, -v1.z, dv2.y, dv0.x);
+    CCC_ACC(c2, -dv1.z, v2.y, dv0.x);
+    CCC_ACC(c2, -dv1.z, dv2.y, v0.x);
+    CCC_ACC(c2, -v2.z, dv0.y, dv1.x);
+    CCC_ACC(c2, -dv2.z, v0.y, dv1.x);
+    CCC_ACC(c2, -dv2.z, dv0.y, v1.x);
+    CCC_ACC(c2, v0.z, dv2.y, dv1.x);
+    CCC_ACC(c2, dv0.z, v2.y, dv1.x);
+    CCC_ACC(c2, dv0.z, dv2.y, v1.x);
+    CCC_ACC(c2, v1.z, dv0.y, dv2.x);
+    CCC_ACC(c2, dv1.z, v0.y, dv2.x);
+    CCC_ACC(c2, dv1.z, dv0.y, v2.x);
+    CCC_ACC(c2, -v0.z, dv1.y, dv2.x);
+    CCC_ACC(c2, -dv0.z, v1.y, dv2.x);
+    CCC_ACC(c2, -dv0.z, dv1.y, v2.x);
+
+    CCC_ACC(c1, v2.z, v1.y, dv0.x);
+    CCC_ACC(c1, v2.z, dv1.y, v0.x);
+    CCC_ACC(c1, dv2.z, v1.y, v0.x);
+    CCC_ACC(c1, -v1.z, v2.y, dv0.x);
+    CCC_ACC(c1, -v1.z, dv2.y, v0.x);
+    CCC_ACC(c1, -dv1.z, v2.y, v0.x);
+    CCC_ACC(c1, -v2.z, v0.y, dv1.x);
+    CCC_ACC(c1, -v2.z, dv0.y, v1.x);
+    CCC_ACC(c1, -dv2.z, v0.y, v1.x);
+    CCC_ACC(c1, v0.z, v2.y, dv1.x);
+    CCC_ACC(c1, v0.z, dv2.y, v1.x);
+    CCC_ACC(c1, dv0.z, v2.y, v1.x);
+    CCC_ACC(c1, v1.z, v0.y, dv2.x);
+    CCC_ACC(c1, v1.z, dv0.y, v2.x);
+    CCC_ACC(c1, dv1.z, v0.y, v2.x);
+    CCC_ACC(c1, -v0.z, v1.y, dv2.x);
+    CCC_ACC(c1, -v0.z, dv1.y, v2.x);
+    CCC_ACC(c1, -dv0.z, v1.y, v2.x);
+
+    CCC_ACC(c0, v2.z, v1.y, v0.x);
+    CCC_ACC(c0, -v1.z, v2.y, v0.x);
+    CCC_ACC(c0, -v2.z, v0.y, v1.x);
+    CCC_ACC(c0, v0.z, v2.y, v1.x);
+    CCC_ACC(c0, v1.z, v0.y, v2.x);
+    CCC_ACC(c0, -v0.z, v1.y, v2.x);
+#undef CCC_ACC
+    c[0] = c3; c[1] = c2; c[2] = c1; c[3] = c0;
+}
+
+// Leading term of the double-double product b*b*b (replaces CAMPARY certifiedMul,
+// collisions_continuous.cu:472-507): TwoProduct via fma (Ogita, Rump & Oishi 2005),
+// then Fast2Sum of the high part with the gathered low-order terms.
+__device__ __forceinline__ double dd_cube_lead(double b) {
+    double p = b * b;
+    double e = fma(b, b, -p);
+    double q = p * b;
+    double f = fma(p, b, -q);
+    double t = f + e * b;
+    return q + t;
+}
+
+// collisions_continuous.cu:448-464 add_root (window [-1e-16, 1+1e-16], slots x,y,z).
+template <typename T> __device__ __forceinline__ void add_root(T *slots, double root, int &n) {
+    if (root < -1e-16 || root > 1 + 1e-16) return;
+    if (n == 0) slots[0] = (T)root;
+    else if (n == 1) slots[1] = (T)root;
+    else if (n == 2) slots[2] = (T)root;
+    n++;
+}
+
+// collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes.
+// slots start at -1; only roots inside the add_root window are stored.
+template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef, T *slots) {
+    const double a = coef[0], b = coef[1], c = coef[2], d = coef[3];
+    int n = 0;
+    slots[0] = slots[1] = slots[2] = (T)-1;
+    if (a == 0 && b == 0) {
+        // :532-536 linear case: the root -d/c is computed by the reference but never stored.
+        return;
+    }
+    if (fabs(a) < 1e-16) {
+        // :537-547 quadratic
+        double D = c * c - 4.0 * b * d;
+        if (D >= 0) {
+            D = sqrt(D);
+            double x1 = (-c + D) / (2.0 * b);
+            double x2 = (-c - D) / (2.0 * b);
+            add_root(slots, x1, n);
+            add_root(slots, x2, n);
+        }
+        return;
+    }
+    const double f = ((3.0 * c / a) - ((b * b) / (a * a))) / 3.0;
+    const double b3 = 2 * dd_cube_lead(b);
+    const double a2 = a * a;
+    const double a3 = dd_cube_lead(a);
+    const double p1 = b3 / a3;
+    const double p2 = (b * c * 9) / a2;
+    const double p3 = 27.0 * d / a;
+    const double g = (p1 - p2 + p3) / 27.0;
+    const double f3 = dd_cube_lead(f);
+    const double h = (g * g) / 4.0 + f3 / 27.0;
+    if (f == 0 && g == 0 && h == 0) {
+        // :564-575 triple root through a float k
+        float k = (float)(d / a);
+        double x1;
+        if (k >= 0) x1 = (double)(cbrtf(k) * -1);
+        else x1 = (double)cbrtf(-k);
+        add_root(slots, x1, n);
+    } else if (h <= 0) {
+        // :576-591 three real roots (trigonometric form)
+        double i = sqrt(((g * g) / 4.0) - h);
+        double j = cbrt(i);
+        double k = acos(-(g / (2 * i)));
+        double L = j * -1;
+        double M = cos(k / 3.0);
+        double N = sqrt(3.0) * sin(k / 3.0);
+        double P = (b / (3.0 * a)) * -1;
+        double x1 = 2 * j * cos(k / 3.0) - (b / (3.0 * a));
+        double x2 = L * (M + N) + P;
+        double x3 = L * (M - N) + P;
+        add_root(slots, x1, n);
+        add_root(slots, x2, n);
+        add_root(slots, x3, n);
+    } else {
+        // :592-611 one real root (Cardano)
+        double R = -(g / 2.0) + sqrt(h);
+        double S = R >= 0 ? cbrt(R) : -cbrt(-R);
+        double V = -(g / 2.0) - sqrt(h);
+        double U = V >= 0 ? cbrt(V) : -cbrt(-V);
+        double x1 = (S + U) - (b / (3.0 * a));
+        add_root(slots, x1, n);
+    }
+}
+
+// Positions at time t: figures.cu:112-117.
+template <typename T> __device__ __forceinline__ void tet_at(const MTet<T> &m, T t, V3<T> *p) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = m.v[k] + m.dv[k] * t;
+}
+
+// collisions_continuous.cu:615-637 is_inside (NaN passes every comparison: accepted).
+template <typename T> __device__ __forceinline__ bool is_inside(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> vt) {
+    V3<T> e1 = v0 - v1, e2 = v2 - v1, ep = vt - v1;
+    T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
+    T denom = d00 * d11 - d01 * d01;
+    T v = (d11 * d20 - d01 * d21) / denom;
+    T w = (d00 * d21 - d01 * d20) / denom;
+    T u = 1 - v - w;
+    if (v < 0 || v > 1) return false;
+    if (w < 0 || w > 1) return false;
+    if (u < 0 || u > 1) return false;
+    return true;
+}
+
+// collisions_continuous.cu:640-684 filterCollisions.
+template <typename T> __device__ __forceinline__ bool filter_collision(const MTet<T> &m, int type, T root) {
+    if (root < -1e-16) return false;
+    V3<T> p[4];
+    tet_at(m, root, p);
+    if (type < 9) {
+        V3<T> da = p[1] - p[0], db = p[3] - p[2], dc = p[2] - p[0];
+        V3<T> cab = cross(da, db);
+        T s = dot(cross(dc, db), cab) / norm2(cab);
+        T q = dot(cross(dc, da), cab) / norm2(cab);
+        return !(s < 0 || s > 1 || q < 0 || q > 1);
+    } else if (type < 12) {
+        return is_inside(p[0], p[1], p[2], p[3]);
+    }
+    return is_inside(p[1], p[2], p[3], p[0]);
+}
+
+// Impulse of one accepted collision: collisions_impulses.cu:24-123 (mass quad),
+// :162-543 (coefficients, normal, normal velocity, mass pair), :936-1050 (magnitudes) and
+// :609-629 (dx extrapolation).  imp[k] / imp_dx[k] go to tetrahedron corner k.
+template <typename T>
+__device__ __forceinline__ void collision_impulse(const MTet<T> &m, int type, T root, const T *q,
+                                                  V3<T> *imp, V3<T> *imp_dx) {
+    V3<T> p[4];
+    tet_at(m, root, p);
+    V3<T> nrm;
+    T nvel, mpx, mpy;
+    if (type < 9) {
+        V3<T> da = p[1] - p[0], db = p[3] - p[2], dc = p[2] - p[0];
+        T denom = norm2(cross(da, db));
+        T s, tt;
+        if (tabs(denom) < 1e-15) { s = (T)0.5; tt = (T)0.5; }
+        else {
+            s = dot(cross(dc, db), cross(da, db)) / norm2(cross(da, db));
+            tt = dot(cross(dc, da), cross(da, db)) / norm2(cross(da, db));
+        }
+        V3<T> p0 = m.v[0] + (m.v[1] - m.v[0]) * s;
+        V3<T> p1 = m.v[2] + (m.v[3] - m.v[2]) * tt;
+        V3<T> dp0 = m.dv[0] + (m.dv[1] - m.dv[0]) * s;
+        V3<T> dp1 = m.dv[2] + (m.dv[3] - m.dv[2]) * tt;
+        V3<T> c0 = m.v[0] + m.dv[0] * root, c1 = m.v[1] + m.dv[1] * root, c2 = m.v[2] + m.dv[2] * root;
+        nrm = normalize(cross(c1 - c0, c2 - c0));
+        if (dot(p1 - p0, nrm) < 0) nrm = nrm * (T)-1;
+        nvel = dot(dp1 - dp0, nrm);
+        mpx = q[0] * (1 - s) + q[1] * s;
+        mpy = q[2] * (1 - tt) + q[3] * tt;
+    } else {
+        const bool inv = type >= 12;
+        const int i0 = inv ? 1 : 0, i1 = inv ? 2 : 1, i2 = inv ? 3 : 2, it = inv ? 0 : 3;
+        V3<T> e1 = p[i0] - p[i1], e2 = p[i2] - p[i1], ep = p[it] - p[i1];
+        T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
+        T denom = d00 * d11 - d01 * d01;
+        T u, w;
+        if (tabs(denom) < 1e-15) { u = (T)(1. / 3.); w = (T)(1. / 3.); }
+        else { u = (d11 * d20 - d01 * d21) / denom; w = (d00 * d21 - d01 * d20) / denom; }
+        T v = 1 - u - w;
+        V3<T> pt = m.v[i0] * u + m.v[i1] * v + m.v[i2] * w;
+        V3<T> dpt = m.dv[i0] * u + m.dv[i1] * v + m.dv[i2] * w;
+        V3<T> c0 = m.v[i0] + m.dv[i0] * root, c1 = m.v[i1] + m.dv[i1] * root, c2 = m.v[i2] + m.dv[i2] * root;
+        nrm = normalize(cross(c1 - c0, c2 - c0));
+        if (dot(m.v[it] - pt, nrm) < 0) nrm = nrm * (T)-1;
+        nvel = dot(m.dv[it] - dpt, nrm);
+        if (inv) nrm = nrm * (T)-1;
+        if (inv) { mpx = q[0]; mpy = q[1] * u + q[2] * v + q[3] * w; }
+        else { mpx = q[0] * u + q[1] * v + q[2] * w; mpy = q[3]; }
+    }
+    T offset;
+    if (tabs(nvel) < 1e-15) offset = (T)0.1;
+    else offset = (T)(2 * 2e-3 / -nvel);
+    offset = (T)fmin(fmax((double)offset, 0.), 1.);
+    T msum = mpx + mpy;
+    T mf0, mf1;
+    if (tabs(msum) < 1e-15) { mf0 = (T)0.5; mf1 = (T)0.5; }
+    else { mf0 = mpx / msum; mf1 = 1 - mf0; }
+    const T im0 = nvel * mf1;
+    const T im1 = -nvel * mf0;
+    const T root_offset = (T)fmax((double)(root - offset), 0.);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        bool second = type < 9 ? (k >= 2) : (type < 12 ? (k == 3) : (k >= 1));
+        imp[k] = nrm * (second ? im1 : im0);
+        V3<T> nvk = m.dv[k] + imp[k];
+        imp_dx[k] = (nvk - m.dv[k]) * (1 - root_offset);
+    }
+}
+
+// proximity_static.cu:205-228 distance_point_to_segment
+template <typename T>
+__device__ __forceinline__ void pt_seg(V3<T> p, V3<T> a, V3<T> b, T *dac, bool set_s, T second) {
+    V3<T> ab = b - a, ap = p - a;
+    T t = dot(ap, ab) / dot(ab, ab);
+    if (t < 0.0) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    V3<T> c = mk(a.x + t * ab.x, a.y + t * ab.y, a.z + t * ab.z);
+    T nd = (T)sqrtf(sq_diff(p, c));
+    if (nd < dac[0]) {
+        dac[0] = nd;
+        if (set_s) { dac[1] = t; dac[2] = second; }
+        else { dac[2] = t; dac[1] = second; }
+    }
+}
+
+// proximity_static.cu:239-332 compute_distance_edge_edge (-1 when the closest points fall
+// outside either segment).
+template <typename T> __device__ __forceinline__ T dist_ee(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> v3) {
+    V3<T> da = v1 - v0, db = v3 - v2, dc = v2 - v0;
+    V3<T> cab = cross(da, db);
+    T s = dot(cross(dc, db), cab) / norm2(cab);
+    T t = dot(cross(dc, da), cab) / norm2(cab);
+    if (isnan(s) || isnan(t)) {
+        T dac[3] = {(T)3.40282346638528859811704183484516925e+38F, (T)-1., (T)-1.};
+        pt_seg(v0, v2, v3, dac, false, (T)0.);
+        pt_seg(v1, v2, v3, dac, false, (T)1.);
+        pt_seg(v2, v0, v1, dac, true, (T)0.);
+        pt_seg(v3, v0, v1, dac, true, (T)1.);
+        s = dac[1];
+        t = dac[2];
+    }
+    bool sv = (s >= 0 && s <= 1), tv = (t >= 0 && t <= 1);
+    if (!sv || !tv) return (T)-1;
+    return (T)sqrtf(sq_diff(v0 + da * s, v2 + db * t));
+}
+
+// proximity_static.cu:334-433 compute_distance_triangle_node
+template <typename T> __device__ __forceinline__ T dist_vf(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> vt) {
+    V3<T> e1 = v0 - v1, e2 = v2 - v1, ep = vt - v1;
+    T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
+    T denom = d00 * d11 - d01 * d01;
+    T u = (d11 * d20 - d01 * d21) / denom;
+    T w = (d00 * d21 - d01 * d20) / denom;
+    T v = 1 - w - u;
+    bool ok = (v >= 0 && v <= 1) && (w >= 0 && w <= 1) && (u >= 0 && u <= 1);
+    if (!ok) return (T)-1;
+    V3<T> pt = v0 * u + v1 * v + v2 * w;
+    return (T)sqrtf(sq_diff(vt, pt));
+}
+
+// figures.cu:378-427 projection interval of a triangle on an axis.
+template <typename T> __device__ __forceinline__ void proj_interval(V3<T> ax, const V3<T> *t, T &lo, T &hi) {
+    T d = dot(ax, t[0]);
+    lo = d; hi = d;
+    d = dot(ax, t[1]); lo = tmin(lo, d); hi = tmax(hi, d);
+    d = dot(ax, t[2]); lo = tmin(lo, d); hi = tmax(hi, d);
+}
+// figures.cu:446-509 17-axis separating-axis test with threshold.
+template <typename T> __device__ __forceinline__ bool sat_overlap(const V3<T> *a, const V3<T> *b, T thr) {
+    V3<T> ae[3] = {a[1] - a[0], a[2] - a[0], a[2] - a[1]};
+    V3<T> be[3] = {b[1] - b[0], b[2] - b[0], b[2] - b[1]};
+    V3<T> an = cross(ae[1], ae[2]), bn = cross(be[1], be[2]);
+    bool flag = true;
+#pragma unroll
+    for (int i = 0; i < 17; ++i) {
+        V3<T> ax;
+        if (i == 0) ax = an;
+        else if (i == 1) ax = bn;
+        else if (i < 11) ax = cross(ae[(i - 2) / 3], be[(i - 2) % 3]);
+        else if (i < 14) ax = cross(an, ae[i - 11]);
+        else ax = cross(an, be[i - 14]);
+        T l1, h1, l2, h2;
+        proj_interval(ax, a, l1, h1);
+        proj_interval(ax, b, l2, h2);
+        flag = flag && ((l1 <= h2 + thr) && (h1 >= l2 - thr));
+    }
+    return flag;
+}
+
+}  // namespace ccc

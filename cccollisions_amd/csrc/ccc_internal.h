@@ -1,0 +1,113 @@
+// Host-side plumbing shared by the C-ABI translation units: context, workspace
+// allocation, error reporting, launch helpers and wave-level primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cccollisions_amd.h"
+
+struct ccc_context {
+    int device;
+    ccc_alloc_fn alloc;
+    ccc_free_fn free_fn;
+    void *user;
+    int num_cus;
+};
+
+namespace ccc {
+
+void set_error(const std::string &msg);
+void clear_error();
+
+// Per-call device workspace.  Buffers come from the context's allocator (or
+// hipMallocAsync on the call's stream) and are released when the Workspace goes out of
+// scope; the call has synchronised its stream before that happens.
+class Workspace {
+  public:
+    Workspace(ccc_context *ctx, hipStream_t s) : ctx_(ctx), s_(s) {}
+    ~Workspace();
+    void *get(size_t bytes);
+    template <typename P> P *get_as(size_t count) { return static_cast<P *>(get(count * sizeof(P))); }
+    bool failed() const { return failed_; }
+
+  private:
+    ccc_context *ctx_;
+    hipStream_t s_;
+    std::vector<void *> ptrs_;
+    bool failed_ = false;
+};
+
+inline ccc_status hip_fail(hipError_t e, const char *what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return CCC_ERR_HIP;
+}
+
+#define CCC_HIP_TRY(expr)                                                                       \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) return ::ccc::hip_fail(_e, #expr);                                \
+    } while (0)
+
+#define CCC_LAUNCH_CHECK(what)                                                                  \
+    do {                                                                                        \
+        hipError_t _e = hipGetLastError();                                                      \
+        if (_e != hipSuccess) return ::ccc::hip_fail(_e, what);                                 \
+    } while (0)
+
+inline ccc_status bad_arg(const std::string &msg) {
+    set_error(msg);
+    return CCC_ERR_BAD_ARG;
+}
+
+inline unsigned grid_for(int64_t items, int block) {
+    int64_t g = (items + block - 1) / block;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+// Device error bits reported by kernels.
+enum : unsigned { DEV_BAD_TREE = 1u, DEV_STACK = 2u };
+
+// Per-batch-element counters kept on the device during a call.
+struct alignas(64) DevCounters {
+    unsigned long long cand;    // K (total, may exceed the candidate capacity)
+    unsigned long long items;   // M
+    unsigned long long roots;   // V
+    unsigned long long coll;    // R (total, may exceed the output capacity)
+    unsigned int flags;         // DEV_* bits
+    unsigned int pad[7];
+};
+
+// Fill kernels (used instead of hipMemsetAsync so that the calls stay capturable).
+ccc_status fill_u64(void *p, int64_t count, uint64_t value, hipStream_t s);
+ccc_status fill_u32(void *p, int64_t count, uint32_t value, hipStream_t s);
+
+}  // namespace ccc
+
+// ---- wave primitives (wave64) ---------------------------------------------------------
+namespace ccc {
+// Number of lanes below the calling lane whose bit is set in `mask`.
+__device__ __forceinline__ unsigned lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+__device__ __forceinline__ unsigned long long readfirstlane_u64(unsigned long long v) {
+    unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Reserve `cnt` consecutive slots (cnt < 16) per active lane of the wave from `counter`
+// with a single atomic; returns this lane's first slot.
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *counter, unsigned cnt) {
+    unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
+    unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2) + 8u * lanes_below(b3);
+    unsigned long long total = (unsigned long long)__popcll(b0) + 2ull * __popcll(b1) + 4ull * __popcll(b2) + 8ull * __popcll(b3);
+    unsigned long long base = 0;
+    if (lanes_below(__ballot(1)) == 0 && total) base = atomicAdd(counter, total);
+    base = readfirstlane_u64(base);
+    return base + below;
+}
+}  // namespace ccc

@@ -1,0 +1,893 @@
+// Queries over the BVH wire format for gfx950: tree preparation, traversal, the 15-way
+// sub-primitive box filter, the fused CCD (coefficients -> cubic -> filter -> collision or
+// impulse) kernel, static proximity, static triangle-triangle SAT and impulses from
+// collision lists.
+//
+// Pipeline of one query call (all batch elements at once, batch index = blockIdx.y):
+//   check tree -> rightmost leaf per internal node -> 64-byte traversal nodes (children's
+//   fp32 boxes rounded outward and grown by the threshold) -> traversal (one lane per
+//   query leaf, LDS stack, exact fp64 leaf test, wave-staged appends) -> candidate pairs
+//   -> per pair: 15-bit sub-primitive mask -> packed (x, y, type) items -> per item:
+//   narrow phase.
+#include <float.h>
+
+#include <algorithm>
+
+#include "ccc_geom.h"
+#include "ccc_internal.h"
+
+namespace ccc {
+
+constexpr int kThreads = 256;
+constexpr int kTravThreads = 256;
+constexpr int kStackDepth = 24;   // pending right children per lane (tree depth - 1)
+constexpr int kStage = 256;       // staged pairs per wave before a flush
+
+struct alignas(16) TNode {
+    float4 a;  // L.min.x, L.min.y, L.min.z, L.max.x
+    float4 b;  // L.max.y, L.max.z, R.min.x, R.min.y
+    float4 c;  // R.min.z, R.max.x, R.max.y, R.max.z
+    int4 d;    // left ref, right ref, rightmost leaf position of left / right child
+};               // ref >= 0: internal row; ref < 0: leaf at position ~ref
+
+__device__ __forceinline__ float f32_down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -INFINITY);
+    return nextafterf(f, -INFINITY);
+}
+__device__ __forceinline__ float f32_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return nextafterf(f, INFINITY);
+}
+
+__device__ __forceinline__ int64_t pack_item(int64_t x, int64_t y, int type) {
+    return (int64_t)(((uint64_t)x << 32) | ((uint64_t)y << 4) | (uint64_t)type);
+}
+__device__ __forceinline__ void unpack_item(int64_t it, int64_t &x, int64_t &y, int &type) {
+    uint64_t u = (uint64_t)it;
+    x = (int64_t)(u >> 32);
+    y = (int64_t)((u & 0xffffffffull) >> 4);
+    type = (int)(u & 15ull);
+}
+
+// Block-wide reservation of `cnt` (< 16) consecutive slots per thread with one atomic.
+// Every thread of the block must call it.
+__device__ __forceinline__ unsigned long long block_reserve(unsigned long long *counter, unsigned cnt) {
+    __shared__ unsigned long long s_wave[kThreads / 64];
+    __shared__ unsigned long long s_base;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
+    unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2) + 8u * lanes_below(b3);
+    unsigned long long wt = (unsigned long long)__popcll(b0) + 2ull * __popcll(b1) + 4ull * __popcll(b2) + 8ull * __popcll(b3);
+    if (lane == 0) s_wave[w] = wt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int q = 0; q < kThreads / 64; ++q) {
+            unsigned long long v = s_wave[q];
+            s_wave[q] = tot;
+            tot += v;
+        }
+        s_base = tot ? atomicAdd(counter, tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long r = s_base + s_wave[w] + below;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void block_add(unsigned long long *counter, unsigned long long v) {
+    __shared__ unsigned long long s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    if (v) atomicAdd(&s_sum, v);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(counter, s_sum);
+}
+
+// ---- tree preparation -------------------------------------------------------------------
+
+// Checks that `tree` is a binary tree over n leaves in the wire format.  Violations set
+// DEV_BAD_TREE; nothing else reads an index before it has been range-checked here.
+__global__ void k_check_tree(const int64_t *__restrict__ tree_all, int64_t n, int *__restrict__ claims_all,
+                             DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t rows = 2 * n - 1;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    int *claims = claims_all + (int64_t)b * n;
+    const int64_t p = tree[4 * r], l = tree[4 * r + 1], rr = tree[4 * r + 2], id = tree[4 * r + 3];
+    bool ok = true;
+    if (n == 1) {
+        ok = p == -1 && l == -1 && rr == -1 && id == 0;
+    } else if (r < n - 1) {
+        ok = l >= 1 && l < rows && rr >= 1 && rr < rows && l != rr;
+        if (ok) ok = tree[4 * l] == r && tree[4 * rr] == r;
+        ok = ok && (r == 0 ? p == -1 : (p >= 0 && p < n - 1));
+    } else {
+        ok = l == -1 && rr == -1 && id >= 0 && id < n && p >= 0 && p < n - 1;
+        if (ok) ok = atomicAdd(&claims[id], 1) == 0;
+    }
+    if (ok && n > 1 && r != 0) ok = tree[4 * p + 1] == r || tree[4 * p + 2] == r;
+    if (!ok) atomicOr(&ctr[b].flags, DEV_BAD_TREE);
+}
+
+// Rightmost leaf position under every internal node (build_bvh.cu:707-743): bottom-up
+// climb, the second thread to arrive at a node combines its children.  Bounded by n steps
+// (a longer climb can only come from a cycle, reported as DEV_BAD_TREE).
+__global__ void k_rightmost(const int64_t *__restrict__ tree_all, int64_t n, int *__restrict__ counters_all,
+                            int *__restrict__ rm_all, DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n || n < 2) return;
+    if (ctr[b].flags & DEV_BAD_TREE) return;
+    const int64_t rows = 2 * n - 1;
+    const int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    int *counters = counters_all + (int64_t)b * (n - 1);
+    int *rm = rm_all + (int64_t)b * (n - 1);
+    int64_t cur = tree[4 * (n - 1 + pos)];
+    for (int64_t step = 0; cur >= 0; ++step) {
+        if (step > n) { atomicOr(&ctr[b].flags, DEV_BAD_TREE); return; }
+        __threadfence();
+        const int old = __hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        __threadfence();
+        const int64_t l = tree[4 * cur + 1], r = tree[4 * cur + 2];
+        const int a = l >= n - 1 ? (int)(l - (n - 1)) : __hip_atomic_load(&rm[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int c = r >= n - 1 ? (int)(r - (n - 1)) : __hip_atomic_load(&rm[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&rm[cur], a > c ? a : c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cur = tree[4 * cur];
+    }
+}
+
+// One 64-byte traversal node per internal row: both children's boxes in fp32, rounded
+// outward, grown by the threshold and by one more float ulp on each side, so that the
+// fp32 test passes whenever the reference's fp64 test passes for either orientation.
+template <typename T>
+__global__ void k_nodes(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all, const int *__restrict__ rm_all,
+                        int64_t n, double thr, TNode *__restrict__ nodes_all, const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (ctr[b].flags & DEV_BAD_TREE) return;
+    const int64_t rows = 2 * n - 1;
+    const T *bb = bb_all + (int64_t)b * rows * 6;
+    const int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    const int *rm = rm_all + (int64_t)b * (n - 1);
+    const int64_t l = tree[4 * i + 1], r = tree[4 * i + 2];
+    float lo[6], ro[6];
+    const T *lb = bb + 6 * l, *rb = bb + 6 * r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = f32_down((double)lb[k] - thr);
+        lo[k + 3] = f32_up((double)lb[k + 3] + thr);
+        ro[k] = f32_down((double)rb[k] - thr);
+        ro[k + 3] = f32_up((double)rb[k + 3] + thr);
+    }
+    TNode nd;
+    nd.a = make_float4(lo[0], lo[1], lo[2], lo[3]);
+    nd.b = make_float4(lo[4], lo[5], ro[0], ro[1]);
+    nd.c = make_float4(ro[2], ro[3], ro[4], ro[5]);
+    const int lref = l >= n - 1 ? ~(int)(l - (n - 1)) : (int)l;
+    const int rref = r >= n - 1 ? ~(int)(r - (n - 1)) : (int)r;
+    const int lrm = l >= n - 1 ? (int)(l - (n - 1)) : rm[l];
+    const int rrm = r >= n - 1 ? (int)(r - (n - 1)) : rm[r];
+    nd.d = make_int4(lref, rref, lrm, rrm);
+    nodes_all[(int64_t)b * (n - 1) + i] = nd;
+}
+
+// ---- traversal ----------------------------------------------------------------------------
+// collisions_static.cu:10-120 (ordering on) and collisions_continuous.cu:127-230,
+// :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
+// reported after the reference's exact test checkOverlap(query box, leaf box, threshold) on
+// the boxes stored in `bboxes`; the pair is (min id, max id).
+template <typename T, bool PARTIAL>
+__global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
+                                                           const TNode *__restrict__ nodes_all, int64_t n, T thr,
+                                                           const uint8_t *__restrict__ to_check_all,
+                                                           int2 *__restrict__ pairs_all, int64_t cap,
+                                                           DevCounters *__restrict__ ctr_all) {
+    __shared__ int stack[kStackDepth][kTravThreads];
+    __shared__ int2 stage[kTravThreads / 64][kStage + 128];
+    __shared__ int fill[kTravThreads / 64];
+    const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
+    const int64_t rows = 2 * n - 1;
+    const T *bb = bb_all + (int64_t)b * rows * 6;
+    const int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    const TNode *nodes = nodes_all + (int64_t)b * (n - 1);
+    DevCounters *ctr = ctr_all + b;
+    int2 *pairs = pairs_all + (int64_t)b * cap;
+    if ((tid & 63) == 0) fill[w] = 0;
+    __syncthreads();
+
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + tid;
+    bool active = p < n && n > 1 && !(ctr->flags & DEV_BAD_TREE);
+    if (PARTIAL && active) active = to_check_all[(int64_t)b * n + p] != 0;
+    const uint8_t *to_check = PARTIAL ? to_check_all + (int64_t)b * n : nullptr;
+
+    if (active) {
+        const int64_t qrow = n - 1 + p;
+        const Box<T> q = ld_box(bb + 6 * qrow);
+        const int qid = (int)tree[4 * qrow + 3];
+        const float qx0 = f32_down((double)q.mn.x), qy0 = f32_down((double)q.mn.y), qz0 = f32_down((double)q.mn.z);
+        const float qx1 = f32_up((double)q.mx.x), qy1 = f32_up((double)q.mx.y), qz1 = f32_up((double)q.mx.z);
+        int sp = 0, node = 0;
+        while (true) {
+            const TNode nd = nodes[node];
+            bool ol = qx0 <= nd.a.w && qx1 >= nd.a.x && qy0 <= nd.b.x && qy1 >= nd.a.y && qz0 <= nd.b.y && qz1 >= nd.a.z;
+            bool orr = qx0 <= nd.c.y && qx1 >= nd.b.z && qy0 <= nd.c.z && qy1 >= nd.b.w && qz0 <= nd.c.w && qz1 >= nd.c.x;
+            if (!PARTIAL) {
+                ol = ol && nd.d.z > (int)p;
+                orr = orr && nd.d.w > (int)p;
+            }
+            int oid0 = -1, oid1 = -1;
+            if (ol && nd.d.x < 0) {
+                const int64_t lrow = n - 1 + (int64_t)(~nd.d.x);
+                if (overlap(q, ld_box(bb + 6 * lrow), thr)) {
+                    const int oid = (int)tree[4 * lrow + 3];
+                    if (!PARTIAL || !to_check[oid] || qid < oid) oid0 = oid;
+                }
+            }
+            if (orr && nd.d.y < 0) {
+                const int64_t rrow = n - 1 + (int64_t)(~nd.d.y);
+                if (overlap(q, ld_box(bb + 6 * rrow), thr)) {
+                    const int oid = (int)tree[4 * rrow + 3];
+                    if (!PARTIAL || !to_check[oid] || qid < oid) oid1 = oid;
+                }
+            }
+            // stage this iteration's pairs in the wave's LDS buffer
+            {
+                const unsigned c = (oid0 >= 0 ? 1u : 0u) + (oid1 >= 0 ? 1u : 0u);
+                const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u);
+                const unsigned below = lanes_below(b0) + 2u * lanes_below(b1);
+                const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1);
+                if (tot) {
+                    const int f = fill[w];
+                    int pos = f + (int)below;
+                    if (oid0 >= 0) stage[w][pos++] = make_int2(min(qid, oid0), max(qid, oid0));
+                    if (oid1 >= 0) stage[w][pos] = make_int2(min(qid, oid1), max(qid, oid1));
+                    const int nf = f + (int)tot;
+                    const unsigned long long act = __ballot(1);
+                    if (nf >= kStage) {
+                        unsigned long long base = 0;
+                        if (lanes_below(act) == 0) base = atomicAdd(&ctr->cand, (unsigned long long)nf);
+                        base = readfirstlane_u64(base);
+                        const int nact = __popcll(act);
+                        for (int j = (int)lanes_below(act); j < nf; j += nact)
+                            if (base + (unsigned long long)j < (unsigned long long)cap) pairs[base + j] = stage[w][j];
+                        if (lanes_below(act) == 0) fill[w] = 0;
+                    } else {
+                        if (lanes_below(act) == 0) fill[w] = nf;
+                    }
+                }
+            }
+            const bool tl = ol && nd.d.x >= 0, tr = orr && nd.d.y >= 0;
+            if (!tl && !tr) {
+                if (sp == 0) break;
+                node = stack[--sp][tid];
+            } else {
+                node = tl ? nd.d.x : nd.d.y;
+                if (tl && tr) {
+                    if (sp >= kStackDepth) { atomicOr(&ctr->flags, DEV_STACK); break; }
+                    stack[sp++][tid] = nd.d.y;
+                }
+            }
+        }
+    }
+    // final flush of the wave's staged pairs (all lanes of the wave take part)
+    const int f = fill[w];
+    if (f > 0) {
+        unsigned long long base = 0;
+        if ((tid & 63) == 0) base = atomicAdd(&ctr->cand, (unsigned long long)f);
+        base = readfirstlane_u64(base);
+        for (int j = tid & 63; j < f; j += 64)
+            if (base + (unsigned long long)j < (unsigned long long)cap) pairs[base + j] = stage[w][j];
+    }
+}
+
+// ---- sub-primitive filter -----------------------------------------------------------------
+// MODE 0: moving (collisions_continuous.cu:234-318, :340-396) -> items.
+// MODE 1: static with threshold (proximity_static.cu:11-118) -> items.
+// MODE 2: moving, written as (x, y, type) rows (find_minimal_candidates, :321-338).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK,
+                                                      const T *__restrict__ t0_all, const T *__restrict__ t1_all,
+                                                      int64_t n, T thr, int64_t *__restrict__ items_all, int64_t capM,
+                                                      DevCounters *__restrict__ ctr_all) {
+    const int b = blockIdx.y;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long kc = ctr->cand;
+    const int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    const int2 *pairs = pairs_all + (int64_t)b * capK;
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    const T *t1 = MODE == 1 ? nullptr : t1_all + (int64_t)b * n * 9;
+    int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
+    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
+        const int64_t i = tile * kThreads + threadIdx.x;
+        uint32_t mask = 0;
+        int x = 0, y = 0;
+        if (i < K) {
+            const int2 pr = pairs[i];
+            x = pr.x;
+            y = pr.y;
+            if (MODE == 1) {
+                V3<T> a[3], c[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
+                if (!share_vertex(a, c)) mask = static_mask(a, c, thr);
+            } else {
+                const MTri<T> a = ld_mtri(t0, t1, x), c = ld_mtri(t0, t1, y);
+                if (!share_vertex(a.v, c.v)) mask = moving_mask(a, c);
+            }
+        }
+        const unsigned cnt = (unsigned)__popc(mask);
+        unsigned long long slot = block_reserve(&ctr->items, cnt);
+        while (mask) {
+            const int type = __ffs(mask) - 1;
+            mask &= mask - 1;
+            if (slot < (unsigned long long)capM) {
+                if (MODE == 2) {
+                    items[3 * slot] = x; items[3 * slot + 1] = y; items[3 * slot + 2] = type;
+                } else {
+                    items[slot] = pack_item(x, y, type);
+                }
+            }
+            ++slot;
+        }
+    }
+}
+
+// ---- fused narrow phase --------------------------------------------------------------------
+// Per (x, y, type) item: tetrahedron (collisions_continuous.cu:36-124), coefficients
+// (:409-445), solveCubic (:511-613), flatten of roots in [0, 1] (:732-765, including
+// duplicates), filterCollisions (:640-684); then
+//   COLLECT: rows (x, y, type) + root, truncated at the output capacity (:1028-1063);
+//   IMPULSE: impulse of the collision scattered with atomics into dv / dx / cnt
+//            (collisions_impulses.cu:1312-1366).
+enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
+
+template <typename T>
+__device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
+                                                T *dv, T *dx, int *cnt) {
+    T q[4];
+    int side[4], idx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        type_corner(type, k, side[k], idx[k]);
+        q[k] = masses[3 * (side[k] ? y : x) + idx[k]];
+    }
+    V3<T> imp[4], imd[4];
+    collision_impulse(m, type, root, q, imp, imd);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t c = 3 * (side[k] ? y : x) + idx[k];
+        atomicAdd(dv + 3 * c + 0, imp[k].x);
+        atomicAdd(dv + 3 * c + 1, imp[k].y);
+        atomicAdd(dv + 3 * c + 2, imp[k].z);
+        atomicAdd(dx + 3 * c + 0, imd[k].x);
+        atomicAdd(dx + 3 * c + 1, imd[k].y);
+        atomicAdd(dx + 3 * c + 2, imd[k].z);
+        atomicAdd(cnt + c, 1);
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+                                                  const T *__restrict__ t0_all, const T *__restrict__ t1_all,
+                                                  const T *__restrict__ masses_all, int64_t n,
+                                                  int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
+                                                  T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
+                                                  DevCounters *__restrict__ ctr_all) {
+    const int b = blockIdx.y;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long mc = ctr->items;
+    const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
+    const int64_t *items = items_all + (int64_t)b * capM;
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    const T *t1 = t1_all + (int64_t)b * n * 9;
+    unsigned long long nroots = 0, ncoll = 0;
+    for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
+        const int64_t i = tile * kThreads + threadIdx.x;
+        int64_t x = 0, y = 0;
+        int type = 0;
+        T acc[3];
+        int nacc = 0;
+        MTet<T> m;
+        if (i < M) {
+            unpack_item(items[i], x, y, type);
+            m = gather_mtet(t0, t1, x, y, type);
+            T cf[4], sl[3];
+            cubic_coeffs(m, cf);
+            solve_cubic(cf, sl);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const T r = sl[s];
+                if (!(r >= 0 && r <= 1)) continue;
+                ++nroots;
+                if (filter_collision(m, type, r)) acc[nacc++] = r;
+            }
+        }
+        if (MODE == MODE_COLLECT) {
+            unsigned long long slot = block_reserve(&ctr->coll, (unsigned)nacc);
+            int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
+            T *roots = roots_all + (int64_t)b * out_cap;
+            for (int k = 0; k < nacc; ++k, ++slot) {
+                if (slot < (unsigned long long)out_cap) {
+                    coll[3 * slot] = x; coll[3 * slot + 1] = y; coll[3 * slot + 2] = type;
+                    roots[slot] = acc[k];
+                }
+            }
+        } else {
+            const T *masses = masses_all + (int64_t)b * n * 3;
+            T *dv = dv_all + (int64_t)b * n * 9;
+            T *dx = dx_all + (int64_t)b * n * 9;
+            int *cnt = cnt_all + (int64_t)b * n * 3;
+            for (int k = 0; k < nacc; ++k) scatter_impulse(m, type, acc[k], x, y, masses, dv, dx, cnt);
+            ncoll += (unsigned long long)nacc;
+        }
+    }
+    block_add(&ctr->roots, nroots);
+    if (MODE == MODE_IMPULSE) block_add(&ctr->coll, ncoll);
+}
+
+// compute_impulses_from_collisions (collisions_impulses.cu:1736-1880) over given rows;
+// rows with a negative or out-of-range index are skipped.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_from_collisions(const int64_t *__restrict__ coll_all, const T *__restrict__ roots_all,
+                                                              int64_t m, const T *__restrict__ t0_all, const T *__restrict__ t1_all,
+                                                              const T *__restrict__ masses_all, int64_t n, T *__restrict__ dv_all,
+                                                              T *__restrict__ dx_all, int *__restrict__ cnt_all) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t *row = coll_all + ((int64_t)b * m + i) * 3;
+    const int64_t x = row[0], y = row[1], type = row[2];
+    if (x < 0 || y < 0 || type < 0 || type > 14 || x >= n || y >= n) return;
+    const T root = roots_all[(int64_t)b * m + i];
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    const T *t1 = t1_all + (int64_t)b * n * 9;
+    const MTet<T> mt = gather_mtet(t0, t1, x, y, (int)type);
+    scatter_impulse(mt, (int)type, root, x, y, masses_all + (int64_t)b * n * 3, dv_all + (int64_t)b * n * 9,
+                    dx_all + (int64_t)b * n * 9, cnt_all + (int64_t)b * n * 3);
+}
+
+// Static proximity per item (proximity_static.cu:435-508): distance by type, kept when
+// 0 <= d <= threshold; rows (x, y, type) truncated at the output capacity.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_proximity(const int64_t *__restrict__ items_all, int64_t capM,
+                                                        const T *__restrict__ t0_all, int64_t n, T thr,
+                                                        int64_t *__restrict__ out_all, int64_t out_cap,
+                                                        DevCounters *__restrict__ ctr_all) {
+    const int b = blockIdx.y;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long mc = ctr->items;
+    const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
+    const int64_t *items = items_all + (int64_t)b * capM;
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    int64_t *out = out_all + (int64_t)b * out_cap * 3;
+    for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
+        const int64_t i = tile * kThreads + threadIdx.x;
+        int64_t x = 0, y = 0;
+        int type = 0;
+        unsigned keep = 0;
+        if (i < M) {
+            unpack_item(items[i], x, y, type);
+            V3<T> p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int side, idx;
+                type_corner(type, k, side, idx);
+                p[k] = ld3(t0 + 9 * (side ? y : x) + 3 * idx);
+            }
+            T d;
+            if (type < 9) d = dist_ee(p[0], p[1], p[2], p[3]);
+            else if (type < 12) d = dist_vf(p[0], p[1], p[2], p[3]);
+            else d = dist_vf(p[1], p[2], p[3], p[0]);
+            keep = (d <= thr && d >= 0) ? 1u : 0u;
+        }
+        const unsigned long long slot = block_reserve(&ctr->coll, keep);
+        if (keep && slot < (unsigned long long)out_cap) {
+            out[3 * slot] = x; out[3 * slot + 1] = y; out[3 * slot + 2] = type;
+        }
+    }
+}
+
+// Static triangle-triangle test per candidate pair (collisions_static.cu:123-159):
+// 17-axis SAT with threshold and no shared vertex; rows (x, y).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sat(const int2 *__restrict__ pairs_all, int64_t capK, const T *__restrict__ t0_all,
+                                                  int64_t n, T thr, int64_t *__restrict__ out_all, int64_t out_cap,
+                                                  DevCounters *__restrict__ ctr_all) {
+    const int b = blockIdx.y;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long kc = ctr->cand;
+    const int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    const int2 *pairs = pairs_all + (int64_t)b * capK;
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    int64_t *out = out_all + (int64_t)b * out_cap * 2;
+    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
+        const int64_t i = tile * kThreads + threadIdx.x;
+        unsigned keep = 0;
+        int x = 0, y = 0;
+        if (i < K) {
+            const int2 pr = pairs[i];
+            x = pr.x;
+            y = pr.y;
+            V3<T> a[3], c[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
+            keep = (sat_overlap(a, c, thr) && !(count_shared(a, c) > 0)) ? 1u : 0u;
+        }
+        const unsigned long long slot = block_reserve(&ctr->coll, keep);
+        if (keep && slot < (unsigned long long)out_cap) { out[2 * slot] = x; out[2 * slot + 1] = y; }
+    }
+}
+
+// ---- host orchestration ---------------------------------------------------------------------
+
+enum class Narrow { SAT, PROXIMITY, CCD_COLLECT, CCD_IMPULSE, MINCOL };
+
+struct Query {
+    ccc_context *ctx;
+    const void *bboxes;
+    const int64_t *tree;
+    const void *t0, *t1, *masses;
+    const uint8_t *to_check;
+    int64_t B, n;
+    int32_t dtype;
+    double thr;             // threshold already converted from the binding's float
+    int64_t cap_candidates; // per batch element (max_candidates * n)
+    int64_t out_cap;        // rows per batch element of the output
+    Narrow narrow;
+    int64_t *out_rows;      // SAT [B,cap,2] / PROXIMITY, CCD, MINCOL [B,cap,3]
+    void *out_roots;        // CCD_COLLECT [B,cap]
+    void *dv, *dx;          // CCD_IMPULSE
+    int32_t *cnt;
+    ccc_counts *counts;
+    uint32_t flags;
+    hipStream_t s;
+};
+
+static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B) {
+    int64_t want = (items + kThreads - 1) / kThreads;
+    int64_t limit = (int64_t)ctx->num_cus * 16 / (B > 0 ? B : 1);
+    if (limit < 4) limit = 4;
+    if (want > limit) want = limit;
+    if (want < 1) want = 1;
+    return (unsigned)want;
+}
+
+template <typename T>
+static ccc_status run_query(const Query &q) {
+    hipStream_t s = q.s;
+    const int64_t B = q.B, n = q.n, rows = 2 * n - 1;
+    const int64_t nint = n > 1 ? n - 1 : 1;
+    Workspace ws(q.ctx, s);
+    DevCounters *ctr = ws.get_as<DevCounters>(B);
+    int *claims = ws.get_as<int>(B * n);
+    int *climb = ws.get_as<int>(B * nint);
+    int *rm = ws.get_as<int>(B * nint);
+    TNode *nodes = ws.get_as<TNode>(B * nint);
+    if (ws.failed()) return CCC_ERR_NO_MEMORY;
+    ccc_status st;
+    if ((st = fill_u64(ctr, B * (int64_t)(sizeof(DevCounters) / 8), 0ull, s)) != CCC_OK) return st;
+    if ((st = fill_u32(claims, B * n, 0u, s)) != CCC_OK) return st;
+    if ((st = fill_u32(climb, B * nint, 0u, s)) != CCC_OK) return st;
+
+    // outputs start at -1 (index rows, roots) or 0 (impulses)
+    switch (q.narrow) {
+        case Narrow::SAT: st = fill_u64(q.out_rows, B * q.out_cap * 2, ~0ull, s); break;
+        case Narrow::PROXIMITY:
+        case Narrow::MINCOL: st = fill_u64(q.out_rows, B * q.out_cap * 3, ~0ull, s); break;
+        case Narrow::CCD_COLLECT:
+            st = fill_u64(q.out_rows, B * q.out_cap * 3, ~0ull, s);
+            if (st == CCC_OK) {
+                if (sizeof(T) == 8) st = fill_u64(q.out_roots, B * q.out_cap, 0xBFF0000000000000ull, s);
+                else st = fill_u32(q.out_roots, B * q.out_cap, 0xBF800000u, s);
+            }
+            break;
+        case Narrow::CCD_IMPULSE:
+            if (sizeof(T) == 8) { st = fill_u64(q.dv, B * n * 9, 0ull, s); if (st == CCC_OK) st = fill_u64(q.dx, B * n * 9, 0ull, s); }
+            else { st = fill_u32(q.dv, B * n * 9, 0u, s); if (st == CCC_OK) st = fill_u32(q.dx, B * n * 9, 0u, s); }
+            if (st == CCC_OK) st = fill_u32(q.cnt, B * n * 3, 0u, s);
+            break;
+    }
+    if (st != CCC_OK) return st;
+
+    const T *bb = (const T *)q.bboxes;
+    hipLaunchKernelGGL(k_check_tree, dim3(grid_for(rows, 256), B), dim3(256), 0, s, q.tree, n, claims, ctr);
+    hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
+    if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
+    CCC_LAUNCH_CHECK("tree preparation");
+
+    // traversal, re-run once with the exact size if the capacity was exceeded
+    std::vector<DevCounters> h(B);
+    int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
+    bool warn_cand = false;
+    int2 *pairs = nullptr;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        pairs = ws.get_as<int2>(B * capK);
+        if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        if (q.to_check)
+            hipLaunchKernelGGL((k_traverse<T, true>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
+                               nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
+        else
+            hipLaunchKernelGGL((k_traverse<T, false>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
+                               nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
+        CCC_LAUNCH_CHECK("k_traverse");
+        CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
+        CCC_HIP_TRY(hipStreamSynchronize(s));
+        unsigned flags = 0;
+        unsigned long long kmax = 0;
+        for (int64_t b = 0; b < B; ++b) {
+            flags |= h[b].flags;
+            kmax = std::max(kmax, h[b].cand);
+        }
+        if (flags & DEV_BAD_TREE) { set_error("tree tensor is not a valid BVH over n leaves"); return CCC_ERR_BAD_TREE; }
+        if (flags & DEV_STACK) { set_error("BVH deeper than the traversal stack"); return CCC_ERR_STACK; }
+        if (kmax <= (unsigned long long)capK) break;
+        warn_cand = true;
+        if (q.flags & CCC_FLAG_FIXED_CAPACITY) break;
+        if (attempt == 1) break;
+        capK = (int64_t)kmax;
+        for (int64_t b = 0; b < B; ++b) h[b].cand = 0;
+        CCC_HIP_TRY(hipMemcpyAsync(ctr, h.data(), sizeof(DevCounters) * B, hipMemcpyHostToDevice, s));
+    }
+    int64_t kmax = 0;
+    for (int64_t b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, std::min<int64_t>((int64_t)h[b].cand, capK));
+
+    const T thr = (T)q.thr;
+    const unsigned gK = persistent_grid(q.ctx, kmax, B);
+    if (q.narrow == Narrow::SAT) {
+        hipLaunchKernelGGL((k_sat<T>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n, thr, q.out_rows,
+                           q.out_cap, ctr);
+        CCC_LAUNCH_CHECK("k_sat");
+    } else if (q.narrow == Narrow::MINCOL) {
+        hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0,
+                           (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr);
+        CCC_LAUNCH_CHECK("k_subprim");
+    } else {
+        const int64_t capM = std::max<int64_t>(15 * kmax, 1);
+        int64_t *items = ws.get_as<int64_t>(B * capM);
+        if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        if (q.narrow == Narrow::PROXIMITY)
+            hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)nullptr,
+                               n, thr, items, capM, ctr);
+        else
+            hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)q.t1,
+                               n, thr, items, capM, ctr);
+        CCC_LAUNCH_CHECK("k_subprim");
+        const unsigned gM = persistent_grid(q.ctx, capM, B);
+        if (q.narrow == Narrow::PROXIMITY)
+            hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr, q.out_rows,
+                               q.out_cap, ctr);
+        else if (q.narrow == Narrow::CCD_COLLECT)
+            hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                               (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap, (T *)nullptr,
+                               (T *)nullptr, (int *)nullptr, ctr);
+        else
+            hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                               (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0, (T *)q.dv,
+                               (T *)q.dx, q.cnt, ctr);
+        CCC_LAUNCH_CHECK("narrow phase");
+    }
+    CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
+    CCC_HIP_TRY(hipStreamSynchronize(s));
+    bool truncated = false;
+    for (int64_t b = 0; b < B; ++b) {
+        const int64_t K = std::min<int64_t>((int64_t)h[b].cand, capK);
+        if (q.counts) {
+            q.counts[b].candidates = (int64_t)h[b].cand;
+            q.counts[b].subprims = (int64_t)h[b].items;
+            q.counts[b].roots = (int64_t)h[b].roots;
+            q.counts[b].collisions = q.narrow == Narrow::MINCOL ? (int64_t)h[b].items : (int64_t)h[b].coll;
+        }
+        (void)K;
+        const int64_t produced = q.narrow == Narrow::MINCOL ? (int64_t)h[b].items : (int64_t)h[b].coll;
+        if (q.narrow != Narrow::CCD_IMPULSE && produced > q.out_cap) truncated = true;
+    }
+    ccc_status out = CCC_OK;
+    if (warn_cand) out |= CCC_WARN_CANDIDATES;
+    if (truncated) out |= CCC_WARN_TRUNCATED;
+    return out;
+}
+
+static ccc_status run(const Query &q) {
+    clear_error();
+    if (!q.ctx) return bad_arg("context is NULL");
+    if (q.B < 0 || q.n < 1) return bad_arg("need batch >= 0 and n >= 1 triangles");
+    if (q.n >= ((int64_t)1 << 28)) return bad_arg("at most 2^28 - 1 triangles per mesh");
+    if (q.B == 0) return CCC_OK;
+    if (!q.bboxes || !q.tree || !q.t0) return bad_arg("NULL tensor pointer");
+    if ((q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE || q.narrow == Narrow::MINCOL) && !q.t1)
+        return bad_arg("triangles_next is NULL");
+    if (q.narrow == Narrow::CCD_IMPULSE && (!q.masses || !q.dv || !q.dx || !q.cnt)) return bad_arg("NULL impulse tensor");
+    if (q.narrow != Narrow::CCD_IMPULSE && (!q.out_rows || q.out_cap < 0)) return bad_arg("NULL output tensor");
+    if (q.narrow == Narrow::CCD_COLLECT && !q.out_roots) return bad_arg("NULL roots tensor");
+    CCC_HIP_TRY(hipSetDevice(q.ctx->device));
+    if (q.dtype == CCC_FLOAT64) return run_query<double>(q);
+    if (q.dtype == CCC_FLOAT32) return run_query<float>(q);
+    return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+}
+
+template <typename T>
+static ccc_status from_collisions(ccc_context *ctx, const int64_t *coll, const void *roots, int64_t m, const void *t0,
+                                  const void *t1, const void *masses, int64_t B, int64_t n, void *dv, void *dx, int32_t *cnt,
+                                  hipStream_t s) {
+    ccc_status st;
+    if (sizeof(T) == 8) { st = fill_u64(dv, B * n * 9, 0ull, s); if (st == CCC_OK) st = fill_u64(dx, B * n * 9, 0ull, s); }
+    else { st = fill_u32(dv, B * n * 9, 0u, s); if (st == CCC_OK) st = fill_u32(dx, B * n * 9, 0u, s); }
+    if (st == CCC_OK) st = fill_u32(cnt, B * n * 3, 0u, s);
+    if (st != CCC_OK) return st;
+    if (m > 0) {
+        hipLaunchKernelGGL((k_from_collisions<T>), dim3(grid_for(m, kThreads), B), dim3(kThreads), 0, s, coll, (const T *)roots, m,
+                           (const T *)t0, (const T *)t1, (const T *)masses, n, (T *)dv, (T *)dx, cnt);
+        CCC_LAUNCH_CHECK("k_from_collisions");
+    }
+    CCC_HIP_TRY(hipStreamSynchronize(s));
+    return CCC_OK;
+}
+
+}  // namespace ccc
+
+using ccc::Narrow;
+using ccc::Query;
+
+static Query base_query(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *t0, int64_t B, int64_t n,
+                        int32_t dtype, uint32_t flags, void *stream) {
+    Query q{};
+    q.ctx = ctx;
+    q.bboxes = bboxes;
+    q.tree = tree;
+    q.t0 = t0;
+    q.B = B;
+    q.n = n;
+    q.dtype = dtype;
+    q.flags = flags;
+    q.s = (hipStream_t)stream;
+    return q;
+}
+
+extern "C" {
+
+ccc_status ccc_find_collisions(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles, int64_t batch,
+                               int64_t n, int32_t dtype, float threshold, int32_t max_collisions, int64_t *out,
+                               ccc_counts *counts, uint32_t flags, void *stream) {
+    Query q = base_query(ctx, bboxes, tree, triangles, batch, n, dtype, flags, stream);
+    q.thr = (double)threshold;
+    q.cap_candidates = (int64_t)max_collisions * n;
+    q.out_cap = (int64_t)max_collisions * n;
+    q.narrow = Narrow::SAT;
+    q.out_rows = out;
+    q.counts = counts;
+    if (max_collisions < 1) return ccc::bad_arg("max_collisions must be >= 1");
+    return ccc::run(q);
+}
+
+ccc_status ccc_find_proximity(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles, int64_t batch,
+                              int64_t n, int32_t dtype, float threshold, int32_t max_candidates, int64_t *out,
+                              ccc_counts *counts, uint32_t flags, void *stream) {
+    Query q = base_query(ctx, bboxes, tree, triangles, batch, n, dtype, flags, stream);
+    q.thr = (double)threshold;
+    q.cap_candidates = (int64_t)max_candidates * n;
+    q.out_cap = (int64_t)max_candidates * n;
+    q.narrow = Narrow::PROXIMITY;
+    q.out_rows = out;
+    q.counts = counts;
+    if (max_candidates < 1) return ccc::bad_arg("max_candidates_per_triangle must be >= 1");
+    return ccc::run(q);
+}
+
+static ccc_status continuous(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *t0, const void *t1,
+                             const uint8_t *to_check, int64_t batch, int64_t n, int32_t dtype, int32_t max_candidates,
+                             int32_t max_collisions, int64_t *collisions, void *roots, ccc_counts *counts, uint32_t flags,
+                             void *stream) {
+    Query q = base_query(ctx, bboxes, tree, t0, batch, n, dtype, flags, stream);
+    q.t1 = t1;
+    q.to_check = to_check;
+    q.thr = 0.0;
+    q.cap_candidates = (int64_t)max_candidates * n;
+    q.out_cap = (int64_t)max_collisions * n;
+    q.narrow = Narrow::CCD_COLLECT;
+    q.out_rows = collisions;
+    q.out_roots = roots;
+    q.counts = counts;
+    if (max_candidates < 1 || max_collisions < 1) return ccc::bad_arg("capacities must be >= 1");
+    return ccc::run(q);
+}
+
+ccc_status ccc_find_collisions_continuous(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                          const void *triangles_next, int64_t batch, int64_t n, int32_t dtype,
+                                          int32_t max_candidates, int32_t max_collisions, int64_t *collisions, void *roots,
+                                          ccc_counts *counts, uint32_t flags, void *stream) {
+    return continuous(ctx, bboxes, tree, triangles, triangles_next, nullptr, batch, n, dtype, max_candidates, max_collisions,
+                      collisions, roots, counts, flags, stream);
+}
+
+ccc_status ccc_find_collisions_continuous_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                                  const void *triangles, const void *triangles_next, const uint8_t *to_check,
+                                                  int64_t batch, int64_t n, int32_t dtype, int32_t max_candidates,
+                                                  int32_t max_collisions, int64_t *collisions, void *roots,
+                                                  ccc_counts *counts, uint32_t flags, void *stream) {
+    if (!to_check) { ccc::set_error("triangles_to_check is NULL"); return CCC_ERR_BAD_ARG; }
+    return continuous(ctx, bboxes, tree, triangles, triangles_next, to_check, batch, n, dtype, max_candidates, max_collisions,
+                      collisions, roots, counts, flags, stream);
+}
+
+ccc_status ccc_find_mincol_cand(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                const void *triangles_next, int64_t batch, int64_t n, int32_t dtype, int32_t max_collisions,
+                                int64_t *out, ccc_counts *counts, uint32_t flags, void *stream) {
+    Query q = base_query(ctx, bboxes, tree, triangles, batch, n, dtype, flags, stream);
+    q.t1 = triangles_next;
+    q.thr = 0.0;
+    q.cap_candidates = (int64_t)max_collisions * n;
+    q.out_cap = (int64_t)max_collisions * n * 15;
+    q.narrow = Narrow::MINCOL;
+    q.out_rows = out;
+    q.counts = counts;
+    if (max_collisions < 1) return ccc::bad_arg("max_collisions must be >= 1");
+    return ccc::run(q);
+}
+
+static ccc_status impulses(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *t0, const void *t1,
+                           const void *masses, const uint8_t *to_check, int64_t batch, int64_t n, int32_t dtype,
+                           int32_t max_candidates, void *dv, void *dx, int32_t *cnt, ccc_counts *counts, uint32_t flags,
+                           void *stream) {
+    Query q = base_query(ctx, bboxes, tree, t0, batch, n, dtype, flags, stream);
+    q.t1 = t1;
+    q.masses = masses;
+    q.to_check = to_check;
+    q.thr = 0.0;
+    q.cap_candidates = (int64_t)max_candidates * n;
+    q.out_cap = 0;
+    q.narrow = Narrow::CCD_IMPULSE;
+    q.dv = dv;
+    q.dx = dx;
+    q.cnt = cnt;
+    q.counts = counts;
+    if (max_candidates < 1) return ccc::bad_arg("max_candidates_per_triangle must be >= 1");
+    return ccc::run(q);
+}
+
+ccc_status ccc_compute_impulses(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                const void *triangles_next, const void *masses, int64_t batch, int64_t n, int32_t dtype,
+                                int32_t max_candidates, int32_t max_collisions, void *dv, void *dx, int32_t *cnt,
+                                ccc_counts *counts, uint32_t flags, void *stream) {
+    (void)max_collisions;
+    return impulses(ctx, bboxes, tree, triangles, triangles_next, masses, nullptr, batch, n, dtype, max_candidates, dv, dx, cnt,
+                    counts, flags, stream);
+}
+
+ccc_status ccc_compute_impulses_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                        const void *triangles_next, const void *masses, const uint8_t *to_check, int64_t batch,
+                                        int64_t n, int32_t dtype, int32_t max_candidates, int32_t max_collisions, void *dv,
+                                        void *dx, int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream) {
+    (void)max_collisions;
+    if (!to_check) { ccc::set_error("triangles_to_check is NULL"); return CCC_ERR_BAD_ARG; }
+    return impulses(ctx, bboxes, tree, triangles, triangles_next, masses, to_check, batch, n, dtype, max_candidates, dv, dx, cnt,
+                    counts, flags, stream);
+}
+
+ccc_status ccc_compute_impulses_from_collisions(ccc_context *ctx, const int64_t *collisions, const void *roots, int64_t m,
+                                                const void *triangles, const void *triangles_next, const void *masses,
+                                                int64_t batch, int64_t n, int32_t dtype, void *dv, void *dx, int32_t *cnt,
+                                                uint32_t flags, void *stream) {
+    (void)flags;
+    ccc::clear_error();
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    if (batch < 0 || n < 1 || m < 0) return ccc::bad_arg("invalid sizes");
+    if (batch == 0) return CCC_OK;
+    if (!triangles || !triangles_next || !masses || !dv || !dx || !cnt || (m > 0 && (!collisions || !roots)))
+        return ccc::bad_arg("NULL tensor pointer");
+    CCC_HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == CCC_FLOAT64)
+        return ccc::from_collisions<double>(ctx, collisions, roots, m, triangles, triangles_next, masses, batch, n, dv, dx, cnt, s);
+    if (dtype == CCC_FLOAT32)
+        return ccc::from_collisions<float>(ctx, collisions, roots, m, triangles, triangles_next, masses, batch, n, dv, dx, cnt, s);
+    return ccc::bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+}
+
+}  // extern "C"

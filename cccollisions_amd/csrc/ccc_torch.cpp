@@ -1,0 +1,381 @@
+// PyTorch-ROCm extension `cccollisions_amd._cccollisions`: the same 14 entry points,
+// argument names, defaults, output shapes and dtypes as the reference bindings
+// (src/cccollisions.cpp:102-468).  Each function checks its inputs, allocates the outputs
+// and calls the C ABI (include/cccollisions_amd.h) on the current HIP stream of the input's
+// device.  Errors raise RuntimeError (the reference exits the process); capacity overflows
+// raise a warning.
+#include <ATen/DeviceGuard.h>
+#include <c10/hip/HIPCachingAllocator.h>
+#include <c10/hip/HIPStream.h>
+#include <torch/extension.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/cccollisions_amd.h"
+
+namespace {
+
+// Reference check macros (src/cccollisions.cpp:98-100) with the same messages.
+#define CHECK_CUDA(x) TORCH_CHECK(x.is_cuda(), #x " must be a CUDA tensor")
+#define CHECK_CONTIGUOUS(x) TORCH_CHECK(x.is_contiguous(), #x " must be contiguous")
+#define CHECK_INPUT(x) \
+    CHECK_CUDA(x);     \
+    CHECK_CONTIGUOUS(x)
+
+void *torch_alloc(void *, size_t bytes, void *stream) {
+    return c10::hip::HIPCachingAllocator::raw_alloc_with_stream(bytes, (hipStream_t)stream);
+}
+void torch_free(void *, void *ptr, void *) { c10::hip::HIPCachingAllocator::raw_delete(ptr); }
+
+ccc_context *context_for(const at::Tensor &t) {
+    static std::mutex mu;
+    static std::vector<ccc_context *> ctxs;
+    const int dev = t.device().index();
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ctxs.size() <= dev) ctxs.resize(dev + 1, nullptr);
+    if (!ctxs[dev]) {
+        ccc_context *c = nullptr;
+        ccc_status st = ccc_context_create(&c, dev, torch_alloc, torch_free, nullptr);
+        TORCH_CHECK(st == CCC_OK, "cccollisions: context creation failed: ", ccc_status_string(st), ": ", ccc_last_error());
+        ctxs[dev] = c;
+    }
+    return ctxs[dev];
+}
+
+void *stream_for(const at::Tensor &t) { return (void *)c10::hip::getCurrentHIPStream(t.device().index()).stream(); }
+
+int32_t dtype_of(const at::Tensor &t) {
+    if (t.scalar_type() == at::kDouble) return CCC_FLOAT64;
+    if (t.scalar_type() == at::kFloat) return CCC_FLOAT32;
+    TORCH_CHECK(false, "cccollisions: triangles must be float32 or float64, got ", t.scalar_type());
+    return -1;
+}
+
+void check_like(const at::Tensor &x, const at::Tensor &ref, const char *name) {
+    TORCH_CHECK(x.scalar_type() == ref.scalar_type(), name, " must have dtype ", ref.scalar_type(), ", got ", x.scalar_type());
+    TORCH_CHECK(x.device() == ref.device(), name, " must be on ", ref.device());
+}
+
+void check_tree(const at::Tensor &bboxes, const at::Tensor &tree, const at::Tensor &triangles) {
+    TORCH_CHECK(bboxes.dim() == 3 && bboxes.size(2) == 6, "bboxes must have shape [B, 2N-1, 6]");
+    TORCH_CHECK(tree.dim() == 3 && tree.size(2) == 4 && tree.size(1) == bboxes.size(1) && tree.size(0) == bboxes.size(0),
+                "tree must have shape [B, 2N-1, 4]");
+    TORCH_CHECK(tree.scalar_type() == at::kLong, "tree must be int64");
+    TORCH_CHECK(bboxes.size(1) % 2 == 1, "bboxes must hold 2N-1 nodes");
+    check_like(bboxes, triangles, "bboxes");
+    TORCH_CHECK(tree.device() == triangles.device(), "tree must be on ", triangles.device());
+}
+
+void check_soup(const at::Tensor &t, int64_t B, int64_t n, const char *name) {
+    TORCH_CHECK(t.dim() == 4 && t.size(0) == B && t.size(1) == n && t.size(2) == 3 && t.size(3) == 3, name,
+                " must have shape [", B, ", ", n, ", 3, 3]");
+}
+
+thread_local uint32_t g_flags = CCC_FLAG_NONE;
+thread_local std::vector<ccc_counts> g_counts;
+
+void finish(ccc_status st, const char *what) {
+    TORCH_CHECK((st & CCC_ERR_MASK) == CCC_OK, "cccollisions.", what, " failed: ", ccc_status_string(st), ": ", ccc_last_error());
+    if (st & CCC_WARN_CANDIDATES) {
+        if (g_flags & CCC_FLAG_FIXED_CAPACITY) {
+            TORCH_WARN("Number of candidate collisions exceeds maximal allowed number. Some of the candidate collisions are omitted.");
+        } else {
+            TORCH_WARN("Number of candidate collisions exceeds max_candidates_per_triangle * N; the candidate buffer was grown to hold all of them.");
+        }
+    }
+    if (st & CCC_WARN_TRUNCATED) {
+        TORCH_WARN("Number of actual collisions exceeds maximal allowed number. Some of the collisions are omitted.");
+    }
+}
+
+ccc_counts *counts_buffer(int64_t B) {
+    g_counts.assign((size_t)B, ccc_counts{0, 0, 0, 0});
+    return g_counts.data();
+}
+
+// ---- entry points (src/cccollisions.cpp) ------------------------------------------------
+
+std::vector<torch::Tensor> bvh_forward(at::Tensor triangles) {
+    CHECK_INPUT(triangles);
+    TORCH_CHECK(triangles.dim() == 4 && triangles.size(2) == 3 && triangles.size(3) == 3, "triangles must have shape [B, N, 3, 3]");
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = triangles.size(0), n = triangles.size(1);
+    TORCH_CHECK(n >= 1, "triangles must hold at least one triangle");
+    auto bboxes = at::empty({B, 2 * n - 1, 6}, triangles.options());
+    auto tree = at::empty({B, 2 * n - 1, 4}, triangles.options().dtype(at::kLong));
+    finish(ccc_bvh(context_for(triangles), triangles.data_ptr(), B, n, dtype_of(triangles), bboxes.data_ptr(),
+                   tree.data_ptr<int64_t>(), stream_for(triangles)),
+           "bvh");
+    return {bboxes, tree};
+}
+
+std::vector<torch::Tensor> bvh_forward_motion(at::Tensor triangles, at::Tensor triangles_next) {
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    TORCH_CHECK(triangles.dim() == 4 && triangles.size(2) == 3 && triangles.size(3) == 3, "triangles must have shape [B, N, 3, 3]");
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = triangles.size(0), n = triangles.size(1);
+    TORCH_CHECK(n >= 1, "triangles must hold at least one triangle");
+    check_soup(triangles_next, B, n, "triangles_next");
+    check_like(triangles_next, triangles, "triangles_next");
+    auto bboxes = at::empty({B, 2 * n - 1, 6}, triangles.options());
+    auto tree = at::empty({B, 2 * n - 1, 4}, triangles.options().dtype(at::kLong));
+    finish(ccc_bvh_motion(context_for(triangles), triangles.data_ptr(), triangles_next.data_ptr(), B, n, dtype_of(triangles),
+                          bboxes.data_ptr(), tree.data_ptr<int64_t>(), stream_for(triangles)),
+           "bvh_motion");
+    return {bboxes, tree};
+}
+
+at::Tensor find_collisions(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, float threshold = 0.,
+                           int max_collisions = 16) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    check_tree(bboxes, tree, triangles);
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    TORCH_CHECK(max_collisions >= 1, "max_collisions must be >= 1");
+    auto out = at::empty({B, n * max_collisions, 2}, bboxes.options().dtype(at::kLong));
+    finish(ccc_find_collisions(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
+                               dtype_of(triangles), threshold, max_collisions, out.data_ptr<int64_t>(), counts_buffer(B),
+                               g_flags, stream_for(triangles)),
+           "find_collisions");
+    return out;
+}
+
+[[noreturn]] void not_in_this_build(const char *name) {
+    TORCH_CHECK_NOT_IMPLEMENTED(false, "cccollisions.", name,
+                                " (triangle-edge intersection queries) is not implemented by cccollisions_amd yet");
+}
+
+std::vector<torch::Tensor> find_triangle_edge_collisions(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
+    not_in_this_build("find_triangle_edge_collisions");
+}
+torch::Tensor find_triangle_edge_candidates(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
+    not_in_this_build("find_triangle_edge_candidates");
+}
+std::vector<torch::Tensor> find_triangle_edge_candidates2(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
+    not_in_this_build("find_triangle_edge_candidates2");
+}
+
+std::vector<torch::Tensor> continuous(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, at::Tensor triangles_next,
+                                      const at::Tensor *to_check, int max_candidates_per_triangle,
+                                      int max_collisions_per_triangle, const char *name) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    check_tree(bboxes, tree, triangles);
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    check_soup(triangles_next, B, n, "triangles_next");
+    check_like(triangles_next, triangles, "triangles_next");
+    TORCH_CHECK(max_candidates_per_triangle >= 1 && max_collisions_per_triangle >= 1, "capacities must be >= 1");
+    const uint8_t *tc = nullptr;
+    if (to_check) {
+        CHECK_INPUT((*to_check));
+        TORCH_CHECK(to_check->scalar_type() == at::kBool && to_check->numel() == B * n, "triangles_to_check must be a [B, N] bool tensor");
+        tc = (const uint8_t *)to_check->data_ptr<bool>();
+    }
+    auto coll = at::empty({B, n * max_collisions_per_triangle, 3}, bboxes.options().dtype(at::kLong));
+    auto roots = at::empty({B, n * max_collisions_per_triangle, 1}, bboxes.options());
+    ccc_status st = tc ? ccc_find_collisions_continuous_partial(
+                             context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(),
+                             triangles_next.data_ptr(), tc, B, n, dtype_of(triangles), max_candidates_per_triangle,
+                             max_collisions_per_triangle, coll.data_ptr<int64_t>(), roots.data_ptr(), counts_buffer(B), g_flags,
+                             stream_for(triangles))
+                       : ccc_find_collisions_continuous(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                                        triangles.data_ptr(), triangles_next.data_ptr(), B, n, dtype_of(triangles),
+                                                        max_candidates_per_triangle, max_collisions_per_triangle,
+                                                        coll.data_ptr<int64_t>(), roots.data_ptr(), counts_buffer(B), g_flags,
+                                                        stream_for(triangles));
+    finish(st, name);
+    return {coll, roots};
+}
+
+std::vector<torch::Tensor> find_collisions_continuous(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles,
+                                                      at::Tensor triangles_next, int max_candidates_per_triangle = 16,
+                                                      int max_collisions_per_triangle = 16) {
+    return continuous(bboxes, tree, triangles, triangles_next, nullptr, max_candidates_per_triangle, max_collisions_per_triangle,
+                      "find_collisions_continuous");
+}
+
+std::vector<torch::Tensor> find_collisions_continuous_partial(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles,
+                                                              at::Tensor triangles_next, at::Tensor triangles_to_check,
+                                                              int max_candidates_per_triangle = 16,
+                                                              int max_collisions_per_triangle = 16) {
+    return continuous(bboxes, tree, triangles, triangles_next, &triangles_to_check, max_candidates_per_triangle,
+                      max_collisions_per_triangle, "find_collisions_continuous_partial");
+}
+
+std::vector<torch::Tensor> find_proximity(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, float threshold,
+                                          int max_candidates_per_triangle = 16) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    check_tree(bboxes, tree, triangles);
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    TORCH_CHECK(max_candidates_per_triangle >= 1, "max_candidates_per_triangle must be >= 1");
+    auto out = at::empty({B, n * max_candidates_per_triangle, 3}, bboxes.options().dtype(at::kLong));
+    finish(ccc_find_proximity(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
+                              dtype_of(triangles), threshold, max_candidates_per_triangle, out.data_ptr<int64_t>(),
+                              counts_buffer(B), g_flags, stream_for(triangles)),
+           "find_proximity");
+    return {out};
+}
+
+torch::Tensor find_mincol_cand(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, at::Tensor triangles_next,
+                               int max_collisions = 16) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    check_tree(bboxes, tree, triangles);
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    check_soup(triangles_next, B, n, "triangles_next");
+    check_like(triangles_next, triangles, "triangles_next");
+    TORCH_CHECK(max_collisions >= 1, "max_collisions must be >= 1");
+    auto out = at::empty({B, n * max_collisions * 15, 3}, bboxes.options().dtype(at::kLong));
+    finish(ccc_find_mincol_cand(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(),
+                                triangles_next.data_ptr(), B, n, dtype_of(triangles), max_collisions, out.data_ptr<int64_t>(),
+                                counts_buffer(B), g_flags, stream_for(triangles)),
+           "find_mincol_cand");
+    return out;
+}
+
+std::vector<torch::Tensor> impulses(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, at::Tensor triangles_next,
+                                    at::Tensor triangles_masses, const at::Tensor *to_check, int max_candidates_per_triangle,
+                                    int max_collisions_per_triangle, const char *name) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    CHECK_INPUT(triangles_masses);
+    check_tree(bboxes, tree, triangles);
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    check_soup(triangles_next, B, n, "triangles_next");
+    check_like(triangles_next, triangles, "triangles_next");
+    check_like(triangles_masses, triangles, "triangles_mass");
+    TORCH_CHECK(triangles_masses.numel() == B * n * 3, "triangles_mass must have shape [B, N, 3]");
+    TORCH_CHECK(max_candidates_per_triangle >= 1, "max_candidates_per_triangle must be >= 1");
+    const uint8_t *tc = nullptr;
+    if (to_check) {
+        CHECK_INPUT((*to_check));
+        TORCH_CHECK(to_check->scalar_type() == at::kBool && to_check->numel() == B * n, "triangles_to_check must be a [B, N] bool tensor");
+        tc = (const uint8_t *)to_check->data_ptr<bool>();
+    }
+    auto dv = at::empty({B, n, 3, 3}, bboxes.options());
+    auto dx = at::empty({B, n, 3, 3}, bboxes.options());
+    auto cnt = at::empty({B, n, 3}, bboxes.options().dtype(at::kInt));
+    ccc_status st = tc ? ccc_compute_impulses_partial(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                                      triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(),
+                                                      tc, B, n, dtype_of(triangles), max_candidates_per_triangle,
+                                                      max_collisions_per_triangle, dv.data_ptr(), dx.data_ptr(),
+                                                      cnt.data_ptr<int32_t>(), counts_buffer(B), g_flags, stream_for(triangles))
+                       : ccc_compute_impulses(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                              triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(), B, n,
+                                              dtype_of(triangles), max_candidates_per_triangle, max_collisions_per_triangle,
+                                              dv.data_ptr(), dx.data_ptr(), cnt.data_ptr<int32_t>(), counts_buffer(B), g_flags,
+                                              stream_for(triangles));
+    finish(st, name);
+    return {dv, dx, cnt};
+}
+
+std::vector<torch::Tensor> compute_impulses(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, at::Tensor triangles_next,
+                                            at::Tensor triangles_masses, int max_candidates_per_triangle = 16,
+                                            int max_collisions_per_triangle = 16) {
+    return impulses(bboxes, tree, triangles, triangles_next, triangles_masses, nullptr, max_candidates_per_triangle,
+                    max_collisions_per_triangle, "compute_impulses");
+}
+
+std::vector<torch::Tensor> compute_impulses_partial(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles,
+                                                    at::Tensor triangles_next, at::Tensor triangles_masses,
+                                                    at::Tensor triangles_to_check, int max_candidates_per_triangle = 16,
+                                                    int max_collisions_per_triangle = 16) {
+    return impulses(bboxes, tree, triangles, triangles_next, triangles_masses, &triangles_to_check, max_candidates_per_triangle,
+                    max_collisions_per_triangle, "compute_impulses_partial");
+}
+
+std::vector<torch::Tensor> compute_impulses_from_collisions(at::Tensor collisions, at::Tensor roots, at::Tensor triangles,
+                                                            at::Tensor triangles_next, at::Tensor triangles_masses) {
+    CHECK_INPUT(collisions);
+    CHECK_INPUT(roots);
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    CHECK_INPUT(triangles_masses);
+    TORCH_CHECK(triangles.dim() == 4 && triangles.size(2) == 3 && triangles.size(3) == 3, "triangles must have shape [B, N, 3, 3]");
+    const at::DeviceGuard guard(triangles.device());
+    const int64_t B = triangles.size(0), n = triangles.size(1);
+    check_soup(triangles_next, B, n, "triangles_next");
+    check_like(triangles_next, triangles, "triangles_next");
+    check_like(triangles_masses, triangles, "triangles_mass");
+    TORCH_CHECK(triangles_masses.numel() == B * n * 3, "triangles_mass must have shape [B, N, 3]");
+    TORCH_CHECK(collisions.scalar_type() == at::kLong && collisions.dim() == 3 && collisions.size(0) == B && collisions.size(2) == 3,
+                "collisions must be an int64 [B, M, 3] tensor");
+    const int64_t m = collisions.size(1);
+    check_like(roots, triangles, "roots");
+    TORCH_CHECK(roots.numel() == B * m, "roots must have shape [B, M, 1]");
+    auto dv = at::empty({B, n, 3, 3}, triangles.options());
+    auto dx = at::empty({B, n, 3, 3}, triangles.options());
+    auto cnt = at::empty({B, n, 3}, triangles.options().dtype(at::kInt));
+    finish(ccc_compute_impulses_from_collisions(context_for(triangles), collisions.data_ptr<int64_t>(), roots.data_ptr(), m,
+                                                triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(), B,
+                                                n, dtype_of(triangles), dv.data_ptr(), dx.data_ptr(), cnt.data_ptr<int32_t>(),
+                                                g_flags, stream_for(triangles)),
+           "compute_impulses_from_collisions");
+    return {dv, dx, cnt};
+}
+
+// ---- extras (not in the reference) -------------------------------------------------------
+void set_fixed_capacity(bool on) { g_flags = on ? (g_flags | CCC_FLAG_FIXED_CAPACITY) : (g_flags & ~CCC_FLAG_FIXED_CAPACITY); }
+
+std::vector<std::vector<int64_t>> last_counts() {
+    std::vector<std::vector<int64_t>> out;
+    for (const auto &c : g_counts) out.push_back({c.candidates, c.subprims, c.roots, c.collisions});
+    return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_cccollisions, m) {
+    m.doc() = "MI355X-native CCCollisions entry points (drop-in for the `cccollisions` extension)";
+    m.def("bvh", &bvh_forward, "BVH over static triangles", py::arg("triangles"));
+    m.def("bvh_motion", &bvh_forward_motion, "BVH over swept triangles", py::arg("triangles"), py::arg("triangles_next"));
+    m.def("find_collisions", &find_collisions, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
+          py::arg("threshold") = 0., py::arg("max_collisions") = 16);
+    m.def("find_triangle_edge_collisions", &find_triangle_edge_collisions, "", py::arg("bboxes"), py::arg("tree"),
+          py::arg("triangles"), py::arg("threshold") = 0., py::arg("max_collisions") = 16);
+    m.def("find_triangle_edge_candidates", &find_triangle_edge_candidates, "", py::arg("bboxes"), py::arg("tree"),
+          py::arg("triangles"), py::arg("threshold") = 0., py::arg("max_collisions") = 16);
+    m.def("find_triangle_edge_candidates2", &find_triangle_edge_candidates2, "", py::arg("bboxes"), py::arg("tree"),
+          py::arg("triangles"), py::arg("threshold") = 0., py::arg("max_collisions") = 16);
+    m.def("find_collisions_continuous", &find_collisions_continuous, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
+          py::arg("triangles_next"), py::arg("max_candidates_per_triangle") = 16, py::arg("max_collisions_per_triangle") = 16);
+    m.def("find_collisions_continuous_partial", &find_collisions_continuous_partial, "", py::arg("bboxes"), py::arg("tree"),
+          py::arg("triangles"), py::arg("triangles_next"), py::arg("triangles_to_check"),
+          py::arg("max_candidates_per_triangle") = 16, py::arg("max_collisions_per_triangle") = 16);
+    m.def("find_proximity", &find_proximity, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"), py::arg("threshold"),
+          py::arg("max_candidates_per_triangle") = 16);
+    m.def("find_mincol_cand", &find_mincol_cand, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
+          py::arg("triangles_next"), py::arg("max_collisions") = 16);
+    m.def("compute_impulses", &compute_impulses, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
+          py::arg("triangles_next"), py::arg("triangles_mass"), py::arg("max_candidates_per_triangle") = 16,
+          py::arg("max_collisions_per_triangle") = 16);
+    m.def("compute_impulses_partial", &compute_impulses_partial, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
+          py::arg("triangles_next"), py::arg("triangles_mass"), py::arg("triangles_to_check"),
+          py::arg("max_candidates_per_triangle") = 16, py::arg("max_collisions_per_triangle") = 16);
+    m.def("compute_impulses_from_collisions", &compute_impulses_from_collisions, "", py::arg("collisions"), py::arg("roots"),
+          py::arg("triangles"), py::arg("triangles_next"), py::arg("triangles_mass"));
+    m.def("set_fixed_capacity", &set_fixed_capacity,
+          "Keep the reference candidate capacity (max_candidates_per_triangle * N) instead of growing it", py::arg("on"));
+    m.def("last_counts", &last_counts, "Per-batch [K, M, V, R] stage counts of the last query on this thread");
+}

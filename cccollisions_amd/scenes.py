@@ -1,0 +1,100 @@
+"""Synthetic cloth scenes used by the benchmark, the smoke check and the parity tests.
+
+Each scene is a vertex/face mesh (numpy, fp64 vertices, int64 faces); the collision
+entry points take triangle soups ``vertices[faces]`` of shape ``[N, 3, 3]``.  All
+randomness comes from ``numpy.random.default_rng(seed)``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class ClothScene:
+    """One simulation state: positions at the start (x0) and end (x1) of a step."""
+    faces: np.ndarray      # [F, 3] int64
+    x0: np.ndarray         # [V, 3] fp64
+    x1: np.ndarray         # [V, 3] fp64
+    vmass: np.ndarray      # [V] fp64 vertex masses
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.faces.shape[0])
+
+    def soups(self):
+        return self.x0[self.faces], self.x1[self.faces]
+
+    def corner_masses(self):
+        return self.vmass[self.faces]
+
+
+def grid_faces(nx: int, ny: int) -> np.ndarray:
+    """Faces of an nx-by-ny vertex grid, two triangles per quad, row-major order."""
+    i, j = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="ij")
+    v00 = (i * ny + j).ravel()
+    v10 = ((i + 1) * ny + j).ravel()
+    v01 = (i * ny + j + 1).ravel()
+    v11 = ((i + 1) * ny + j + 1).ravel()
+    f = np.empty((v00.size * 2, 3), np.int64)
+    f[0::2] = np.stack([v00, v10, v11], 1)
+    f[1::2] = np.stack([v00, v11, v01], 1)
+    return f
+
+
+def grid_vertices(nx: int, ny: int) -> np.ndarray:
+    x, y = np.meshgrid(np.linspace(0.0, 1.0, nx), np.linspace(0.0, 1.0, ny), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), np.zeros(nx * ny)], 1)
+
+
+def proximity_cloth(n: int = 64, seed: int = 1, jitter: float = 1e-3) -> tuple[np.ndarray, np.ndarray]:
+    """Static cloth: n-by-n vertex grid on the unit square,
+    z = 0.05 sin(6x) cos(6y) + N(0, jitter).  Returns (vertices, faces)."""
+    rng = np.random.default_rng(seed)
+    v = grid_vertices(n, n)
+    v[:, 2] = 0.05 * np.sin(6 * v[:, 0]) * np.cos(6 * v[:, 1]) + rng.normal(0.0, jitter, v.shape[0])
+    return v, grid_faces(n, n)
+
+
+def folded_cloth(n: int = 256, seed: int = 2, gap: float = 1e-3, sigma: float = 0.5,
+                 dt: float = 1.0 / 30.0) -> ClothScene:
+    """Cloth folded over itself: n-by-n vertex grid on the unit square, vertices with
+    x > 0.5 mirrored to (1 - x, y, gap); t1 = t0 + dt * v with v ~ N(0, sigma) per
+    vertex component.  Vertex masses are 1."""
+    rng = np.random.default_rng(seed)
+    v = grid_vertices(n, n)
+    right = v[:, 0] > 0.5
+    v[right, 0] = 1.0 - v[right, 0]
+    v[right, 2] = gap
+    vel = rng.normal(0.0, sigma, v.shape)
+    return ClothScene(grid_faces(n, n), v, v + dt * vel, np.ones(v.shape[0]))
+
+
+def grid_size_for_triangles(num_tris: int) -> int:
+    """Vertex count per side of the square grid with about num_tris triangles."""
+    return max(2, int(round(math.sqrt(num_tris / 2.0))) + 1)
+
+
+def batched_cloths(count: int = 64, lo: int = 2000, hi: int = 200000, seed: int = 4,
+                   **kw) -> list[ClothScene]:
+    """Independent folded cloths whose triangle counts are log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), count))
+    return [folded_cloth(grid_size_for_triangles(int(s)), seed=seed * 1000 + i, **kw)
+            for i, s in enumerate(sizes)]
+
+
+def apply_corrections(faces: np.ndarray, x1: np.ndarray, dx: np.ndarray, cnt: np.ndarray) -> np.ndarray:
+    """Caller-side update after one impulse pass: every vertex moves by the mean of the
+    position corrections its triangle corners received (corners with cnt = 0 add
+    nothing).  Returns the new end-of-step positions."""
+    vs = np.zeros_like(x1)
+    vc = np.zeros(x1.shape[0])
+    np.add.at(vs, faces.reshape(-1), dx.reshape(-1, 3))
+    np.add.at(vc, faces.reshape(-1), cnt.reshape(-1).astype(np.float64))
+    out = x1.copy()
+    hit = vc > 0
+    out[hit] += vs[hit] / vc[hit, None]
+    return out

@@ -1,0 +1,166 @@
+/*
+ * cccollisions_amd -- C ABI of the MI355X (gfx950) collision hot path.
+ *
+ * Drop-in replacement for the host functions behind the `cccollisions` PyTorch
+ * extension of Dolorousrtur/CCCollisions.  Each entry point below replaces one host
+ * orchestrator of the reference (cited per function) and keeps its argument meaning,
+ * capacities and output layouts; the torch extension (cccollisions_amd/csrc/ccc_torch.cpp)
+ * is the counterpart of src/cccollisions.cpp and calls only these functions.
+ *
+ * Conventions
+ *   - Plain device pointers and sizes; no torch types.  Every tensor is contiguous,
+ *     row-major, on the device the context was created for.
+ *   - Triangle soups: [batch, n, 3, 3] of `dtype` (vertex-major: tri, corner, xyz).
+ *   - BVH wire format: bboxes [batch, 2n-1, 6] (min xyz, max xyz) of `dtype`,
+ *     tree [batch, 2n-1, 4] int64 (parent, left, right, triangle id); internal nodes are
+ *     rows 0..n-2 (row 0 is the root), leaves are rows n-1..2n-2 in sorted order.
+ *   - Outputs are caller-allocated and fully written by the call: index rows and roots
+ *     are padded with -1, impulse sums and counts start from zero.
+ *   - All work is enqueued on `stream` (a hipStream_t; NULL = default stream).  A call
+ *     returns after the results it reports (counts, warnings) are known, i.e. after a
+ *     stream synchronisation.
+ *   - Returns a ccc_status: low byte = error code (0 = success), upper bits = warnings.
+ *   - A context holds no per-call state; calls on one context from several host threads
+ *     are safe as long as they use different streams.
+ */
+#ifndef CCCOLLISIONS_AMD_H
+#define CCCOLLISIONS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCC_ABI_VERSION 1
+
+typedef int32_t ccc_status;
+#define CCC_OK 0
+#define CCC_ERR_BAD_ARG 1          /* invalid size, pointer or dtype                     */
+#define CCC_ERR_HIP 2              /* a HIP runtime call or kernel launch failed          */
+#define CCC_ERR_NO_MEMORY 3        /* workspace allocation failed                         */
+#define CCC_ERR_BAD_TREE 4         /* tree tensor is not a binary tree over n leaves      */
+#define CCC_ERR_STACK 5            /* traversal deeper than the stack (tree too deep)     */
+#define CCC_ERR_MASK 0xff
+#define CCC_WARN_CANDIDATES 0x100  /* more candidate pairs than max_candidates * n        */
+#define CCC_WARN_TRUNCATED 0x200   /* more result rows than the output holds; rest dropped */
+
+typedef enum { CCC_FLOAT32 = 0, CCC_FLOAT64 = 1 } ccc_dtype;
+
+/* Call flags. */
+#define CCC_FLAG_NONE 0u
+/* Keep the reference capacity for candidate pairs (max_candidates * n): pairs beyond it
+ * are dropped and CCC_WARN_CANDIDATES is raised.  Without this flag the candidate buffer
+ * grows to the exact count and the warning still reports that the capacity was exceeded. */
+#define CCC_FLAG_FIXED_CAPACITY 1u
+
+/* Per-batch-element stage counts. */
+typedef struct {
+    int64_t candidates;  /* K: triangle pairs from the broad phase                        */
+    int64_t subprims;    /* M: vertex-face / edge-edge pairs after the 15-way box filter */
+    int64_t roots;       /* V: cubic roots in [0, 1]                                     */
+    int64_t collisions;  /* R: accepted collisions / close pairs / intersecting pairs    */
+} ccc_counts;
+
+/* Device workspace allocator (e.g. a framework's caching allocator).  Both callbacks are
+ * optional; without them the library uses hipMallocAsync / hipFreeAsync on the stream. */
+typedef void *(*ccc_alloc_fn)(void *user, size_t bytes, void *stream);
+typedef void (*ccc_free_fn)(void *user, void *ptr, void *stream);
+
+typedef struct ccc_context ccc_context;
+
+ccc_status ccc_context_create(ccc_context **out, int device, ccc_alloc_fn alloc, ccc_free_fn free_fn,
+                              void *user);
+void ccc_context_destroy(ccc_context *ctx);
+const char *ccc_status_string(ccc_status status);
+/* Message of the last failed call made on this host thread (empty if none). */
+const char *ccc_last_error(void);
+int ccc_abi_version(void);
+
+/* Replaces build_bvh(triangles, ...) -- src/build_bvh.cu:381-449 (binding
+ * src/cccollisions.cpp:102-116).  Static leaf boxes. */
+ccc_status ccc_bvh(ccc_context *ctx, const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                   void *bboxes, int64_t *tree, void *stream);
+
+/* Replaces build_bvh(triangles, triangles_next, ...) -- src/build_bvh.cu:570-648 (binding
+ * src/cccollisions.cpp:118-132).  Swept leaf boxes box(t0) U box(t1). */
+ccc_status ccc_bvh_motion(ccc_context *ctx, const void *triangles, const void *triangles_next,
+                          int64_t batch, int64_t n, int32_t dtype, void *bboxes, int64_t *tree,
+                          void *stream);
+
+/* Replaces find_collisions_from_bbox_tree(bboxes, tree, triangles, collisions, threshold,
+ * max_collisions) -- src/collisions_static.cu:206-342 (binding src/cccollisions.cpp:134-148).
+ * out: [batch, n*max_collisions, 2] int64 intersecting pairs (x < y). */
+ccc_status ccc_find_collisions(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                               const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                               float threshold, int32_t max_collisions, int64_t *out,
+                               ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces find_proximity_from_bbox_tree -- src/proximity_static.cu:510-695 (binding
+ * src/cccollisions.cpp:276-296).  out: [batch, n*max_candidates, 3] int64 (x, y, type). */
+ccc_status ccc_find_proximity(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                              const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                              float threshold, int32_t max_candidates, int64_t *out,
+                              ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces find_collisions_from_bbox_tree(..., triangles_next, collisions, roots, ...) --
+ * src/collisions_continuous.cu:814-1076 (binding src/cccollisions.cpp:220-245).
+ * collisions: [batch, n*max_collisions, 3] int64 (x, y, type); roots: [batch, n*max_collisions]. */
+ccc_status ccc_find_collisions_continuous(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                          const void *triangles, const void *triangles_next,
+                                          int64_t batch, int64_t n, int32_t dtype,
+                                          int32_t max_candidates, int32_t max_collisions,
+                                          int64_t *collisions, void *roots, ccc_counts *counts,
+                                          uint32_t flags, void *stream);
+
+/* Replaces find_collisions_from_bbox_tree_partial -- src/collisions_continuous.cu:1103-1376
+ * (binding src/cccollisions.cpp:247-273).  to_check: [batch, n] bool (1 byte each). */
+ccc_status ccc_find_collisions_continuous_partial(ccc_context *ctx, const void *bboxes,
+                                                  const int64_t *tree, const void *triangles,
+                                                  const void *triangles_next, const uint8_t *to_check,
+                                                  int64_t batch, int64_t n, int32_t dtype,
+                                                  int32_t max_candidates, int32_t max_collisions,
+                                                  int64_t *collisions, void *roots,
+                                                  ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces find_minimal_candidates_from_bbox_tree -- src/collisions_continuous.cu:1379-1519
+ * (binding src/cccollisions.cpp:405-423).  out: [batch, n*max_collisions*15, 3] int64. */
+ccc_status ccc_find_mincol_cand(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                const void *triangles, const void *triangles_next, int64_t batch,
+                                int64_t n, int32_t dtype, int32_t max_collisions, int64_t *out,
+                                ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces collision_impulses_from_bbox_tree -- src/collisions_impulses.cu:1069-1397
+ * (binding src/cccollisions.cpp:299-333).  masses: [batch, n, 3]; dv, dx: [batch, n, 3, 3]
+ * of dtype; cnt: [batch, n, 3] int32.  max_collisions is accepted and unused, as in the
+ * reference. */
+ccc_status ccc_compute_impulses(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                const void *triangles, const void *triangles_next,
+                                const void *masses, int64_t batch, int64_t n, int32_t dtype,
+                                int32_t max_candidates, int32_t max_collisions, void *dv, void *dx,
+                                int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces collision_impulses_from_bbox_tree_partial -- src/collisions_impulses.cu:1402-1734
+ * (binding src/cccollisions.cpp:336-370). */
+ccc_status ccc_compute_impulses_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                        const void *triangles, const void *triangles_next,
+                                        const void *masses, const uint8_t *to_check, int64_t batch,
+                                        int64_t n, int32_t dtype, int32_t max_candidates,
+                                        int32_t max_collisions, void *dv, void *dx, int32_t *cnt,
+                                        ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces collision_impulses_from_continuous_collisions -- src/collisions_impulses.cu:1736-1880
+ * (binding src/cccollisions.cpp:374-403).  collisions: [batch, m, 3] int64; roots:
+ * [batch, m] of dtype.  Rows holding a negative index (the -1 padding) are skipped. */
+ccc_status ccc_compute_impulses_from_collisions(ccc_context *ctx, const int64_t *collisions,
+                                                const void *roots, int64_t m,
+                                                const void *triangles, const void *triangles_next,
+                                                const void *masses, int64_t batch, int64_t n,
+                                                int32_t dtype, void *dv, void *dx, int32_t *cnt,
+                                                uint32_t flags, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCCOLLISIONS_AMD_H */

@@ -1,0 +1,252 @@
+"""Parity of the HIP path (through the torch extension, which calls the C ABI) against the
+CPU oracle on seeded inputs.  Index / pair / collision lists are compared bit-exactly as
+sorted multisets; roots and impulses within 1e-9 relative."""
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.conftest import sorted_rows, valid_rows
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def _t(a, dev="cuda"):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _check_close(gpu, ref, name):
+    scale = max(float(np.abs(ref).max()) if ref.size else 0.0, 1e-300)
+    err = np.abs(gpu - ref)
+    bad = err > RTOL * np.abs(ref) + 1e-12 * scale
+    assert not bad.any(), f"{name}: {int(bad.sum())} entries differ, max err {err.max():.3e} (scale {scale:.3e})"
+
+
+SCENES = {
+    "folded20": lambda: cases.folded(20),
+    "random300": lambda: cases.random_soup(300),
+    "shared200": lambda: cases.shared_vertex_soup(200),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("motion", [False, True])
+def test_bvh_bit_exact(ccc, oracle, name, motion):
+    t0, t1, _ = SCENES[name]()
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None])) if motion else ccc.bvh(_t(t0[None]))
+    rb, rt, codes, ids = oracle.bvh(t0, t1 if motion else None)
+    assert (codes == 0).all()
+    np.testing.assert_array_equal(_np(tree)[0], rt)
+    np.testing.assert_array_equal(_np(bb)[0], rb)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 1000, 4097])
+def test_bvh_sizes(ccc, oracle, n):
+    t0, t1, _ = cases.random_soup(n, seed=n)
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    np.testing.assert_array_equal(_np(tree)[0], rt)
+    np.testing.assert_array_equal(_np(bb)[0], rb)
+
+
+def test_bvh_batch_and_fp32(ccc, oracle):
+    a = cases.random_soup(257, seed=1, dtype=np.float32)
+    b = cases.random_soup(257, seed=2, dtype=np.float32)
+    t0 = np.stack([a[0], b[0]])
+    t1 = np.stack([a[1], b[1]])
+    bb, tree = ccc.bvh_motion(_t(t0), _t(t1))
+    for i in range(2):
+        rb, rt, _, _ = oracle.bvh(t0[i], t1[i])
+        np.testing.assert_array_equal(_np(tree)[i], rt)
+        np.testing.assert_array_equal(_np(bb)[i], rb)
+
+
+def _ccd_ref(oracle, t0, t1):
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    pairs, _ = oracle.traverse(rb, rt, 0.0)
+    coll, roots, counts = oracle.ccd(t0, t1, pairs)
+    return rb, rt, coll, roots, counts
+
+
+def _compare_collisions(gc, gr, coll, roots):
+    g = valid_rows(gc)
+    gr = gr[: g.shape[0]].reshape(-1)
+    assert g.shape[0] == coll.shape[0], f"{g.shape[0]} GPU collisions vs {coll.shape[0]} oracle"
+    if coll.shape[0] == 0:
+        return
+    ka = np.column_stack([g, gr])
+    kb = np.column_stack([coll, roots])
+    ia = np.lexsort((gr, g[:, 2], g[:, 1], g[:, 0]))
+    ib = np.lexsort((roots, coll[:, 2], coll[:, 1], coll[:, 0]))
+    np.testing.assert_array_equal(ka[ia][:, :3].astype(np.int64), kb[ib][:, :3].astype(np.int64))
+    _check_close(ka[ia][:, 3], kb[ib][:, 3], "roots")
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_continuous_collisions(ccc, oracle, name):
+    t0, t1, _ = SCENES[name]()
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    mc = int(np.ceil(counts["candidates"] / len(t0))) + 1
+    mcol = int(np.ceil(max(counts["collisions"], 1) / len(t0))) + 1
+    gc, gr = ccc.find_collisions_continuous(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), mc, mcol)
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, V, R) == (counts["candidates"], counts["subprims"], counts["roots"], counts["collisions"])
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_compute_impulses(ccc, oracle, name):
+    t0, t1, m = SCENES[name]()
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    dv, dx, cnt = oracle.impulses(t0, t1, m, coll, roots)
+    mc = int(np.ceil(counts["candidates"] / len(t0))) + 1
+    g = ccc.compute_impulses(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), _t(m[None]), mc, 16)
+    assert ccc.last_counts()[0][3] == counts["collisions"]
+    np.testing.assert_array_equal(_np(g[2])[0], cnt)
+    _check_close(_np(g[0])[0], dv, "dv")
+    _check_close(_np(g[1])[0], dx, "dx")
+
+
+def test_impulses_from_collisions_with_padding(ccc, oracle):
+    t0, t1, m = cases.folded(20)
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    cap = coll.shape[0] + 17
+    c = -np.ones((1, cap, 3), np.int64)
+    r = -np.ones((1, cap, 1))
+    c[0, : coll.shape[0]] = coll
+    r[0, : coll.shape[0], 0] = roots
+    g = ccc.compute_impulses_from_collisions(_t(c), _t(r), _t(t0[None]), _t(t1[None]), _t(m[None]))
+    dv, dx, cnt = oracle.impulses(t0, t1, m, coll, roots)
+    np.testing.assert_array_equal(_np(g[2])[0], cnt)
+    _check_close(_np(g[0])[0], dv, "dv")
+    _check_close(_np(g[1])[0], dx, "dx")
+
+
+def test_candidate_capacity_grows(ccc, oracle):
+    t0, t1, _ = cases.folded(20)
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    assert counts["candidates"] > len(t0)          # default capacity 1 per triangle overflows
+    with pytest.warns(UserWarning):
+        gc, gr = ccc.find_collisions_continuous(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), 1, 64)
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+
+
+def test_output_truncation_warns(ccc, oracle):
+    t0, t1, _ = cases.folded(20)
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    assert counts["collisions"] > len(t0)
+    with pytest.warns(UserWarning):
+        gc, gr = ccc.find_collisions_continuous(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), 64, 1)
+    g = valid_rows(_np(gc)[0])
+    assert g.shape[0] == len(t0)
+    ref = {tuple(r) for r in coll}
+    assert all(tuple(r) in ref for r in g)
+
+
+def test_batched_continuous(ccc, oracle):
+    a = cases.random_soup(300, seed=11)
+    b = cases.random_soup(300, seed=12)
+    t0 = np.stack([a[0], b[0]])
+    t1 = np.stack([a[1], b[1]])
+    bb, tree = ccc.bvh_motion(_t(t0), _t(t1))
+    gc, gr = ccc.find_collisions_continuous(bb, tree, _t(t0), _t(t1), 64, 64)
+    for i in range(2):
+        _, _, coll, roots, _ = _ccd_ref(oracle, t0[i], t1[i])
+        _compare_collisions(_np(gc)[i], _np(gr)[i], coll, roots)
+
+
+def test_fp32_continuous(ccc, oracle):
+    t0, t1, m = cases.random_soup(300, seed=7, dtype=np.float32)
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    gc, gr = ccc.find_collisions_continuous(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), 64, 64)
+    g = sorted_rows(valid_rows(_np(gc)[0]))
+    np.testing.assert_array_equal(g, sorted_rows(coll))
+
+
+def test_partial(ccc, oracle):
+    t0, t1, m = cases.random_soup(300, seed=9)
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    rng = np.random.default_rng(0)
+    mask = rng.random(300) < 0.3
+    pairs = oracle.traverse_partial(rb, rt, mask)
+    coll, roots, counts = oracle.ccd(t0, t1, pairs)
+    import torch
+    tc = torch.from_numpy(mask[None].copy()).cuda()
+    gc, gr = ccc.find_collisions_continuous_partial(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), tc, 64, 64)
+    assert ccc.last_counts()[0][0] == counts["candidates"]
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+    dv, dx, cnt = oracle.impulses(t0, t1, m, coll, roots)
+    g = ccc.compute_impulses_partial(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), _t(m[None]), tc, 64, 16)
+    np.testing.assert_array_equal(_np(g[2])[0], cnt)
+    _check_close(_np(g[0])[0], dv, "dv")
+
+
+@pytest.mark.parametrize("gap", [1e-3, 5e-3])
+def test_proximity(ccc, oracle, gap):
+    tri = cases.proximity_fold(24, gap=gap)
+    rb, rt, _, _ = oracle.bvh(tri)
+    rows, counts = oracle.find_proximity(rb, rt, tri, 2e-3)
+    mc = int(np.ceil(max(counts["candidates"], counts["collisions"]) / len(tri))) + 1
+    g = ccc.find_proximity(_t(rb[None]), _t(rt[None]), _t(tri[None]), 2e-3, mc)[0]
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, R) == (counts["candidates"], counts["subprims"], counts["collisions"])
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(g)[0])), sorted_rows(rows))
+
+
+def test_proximity_config1_shape(ccc, oracle):
+    from cccollisions_amd import scenes
+    v, f = scenes.proximity_cloth(64, seed=1)
+    tri = v[f]
+    bb, tree = ccc.bvh(_t(tri[None]))
+    rb, rt, _, _ = oracle.bvh(tri)
+    np.testing.assert_array_equal(_np(tree)[0], rt)
+    rows, counts = oracle.find_proximity(rb, rt, tri, 2e-3)
+    g = ccc.find_proximity(bb, tree, _t(tri[None]), 2e-3, 16)[0]
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, R) == (counts["candidates"], counts["subprims"], counts["collisions"])
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(g)[0])), sorted_rows(rows))
+
+
+@pytest.mark.parametrize("thr", [0.0, 1e-3])
+def test_static_sat(ccc, oracle, thr):
+    t0, _, _ = cases.random_soup(400, seed=4)
+    rb, rt, _, _ = oracle.bvh(t0)
+    ref = oracle.find_collisions(rb, rt, t0, thr)
+    g = ccc.find_collisions(_t(rb[None]), _t(rt[None]), _t(t0[None]), thr, 64)
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(g)[0])), sorted_rows(ref))
+
+
+def test_mincol(ccc, oracle):
+    t0, t1, _ = cases.random_soup(300, seed=3)
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    pairs, _ = oracle.traverse(rb, rt, 0.0)
+    ref = oracle.mincol(t0, t1, pairs)
+    mc = int(np.ceil(pairs.shape[0] / 300)) + 1
+    g = ccc.find_mincol_cand(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), mc)
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(g)[0])), sorted_rows(ref))
+
+
+def test_static_pairs_have_no_roots(ccc, oracle):
+    """t1 == t0: every cubic is c3 = c2 = c1 = 0 and falls in the linear branch, whose root
+    the reference never stores (collisions_continuous.cu:532-536)."""
+    t0, _, m = cases.folded(16)
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t0[None]))
+    gc, gr = ccc.find_collisions_continuous(bb, tree, _t(t0[None]), _t(t0[None]), 64, 16)
+    K, M, V, R = ccc.last_counts()[0]
+    assert K > 0 and M > 0 and V == 0 and R == 0
+    assert (_np(gc) == -1).all() and (_np(gr) == -1).all()
+
+
+def test_bad_tree_raises(ccc, oracle):
+    t0, t1, _ = cases.random_soup(50, seed=1)
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    bad = rt.copy()
+    bad[3, 1] = 5     # child whose parent field does not point back
+    with pytest.raises(RuntimeError, match="malformed BVH tree"):
+        ccc.find_collisions_continuous(_t(rb[None]), _t(bad[None]), _t(t0[None]), _t(t1[None]), 64, 16)
