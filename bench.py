@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark: CCD + impulse steps per second on the 130k-triangle folded cloth.
+
+Workload (one "step"): a 256x256-vertex cloth (130,050 triangles, fp64) folded over itself
+at x = 0.5 with a 1e-3 gap, end-of-step positions t1 = t0 + v/30 with v ~ N(0, 0.5)
+(seeded).  A step runs 10 impulse iterations; each iteration gathers the triangle soups,
+builds the swept BVH (`bvh_motion`), runs `compute_impulses` (traversal -> 15-way VF/EE
+filter -> cubic CCD -> impulse scatter) and moves every vertex of t1 by the mean of the
+finite position corrections of its triangle corners (scenes.apply_corrections).  Inputs are resident in HBM before timing.
+
+Prints one JSON line: throughput, per-step latency, the roofline of the dominant kernel
+(its duration measured with HIP events on the stream it runs on) and a CPU baseline (the
+oracle's OpenMP pass on a bounded sample of the same workload, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CCD+impulse steps/sec at 130k tris (batched vs plain); p99 request latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+FLOPS_PER_SUBPRIM = 162          # cubic-coefficient construction per VF/EE pair
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=256, help="vertices per side of the cloth grid")
+    ap.add_argument("--iters", type=int, default=10, help="impulse iterations per step")
+    ap.add_argument("--max-candidates", type=int, default=512)
+    ap.add_argument("--cpu-stride", type=int, default=2,
+                    help="CPU baseline runs every n-th query leaf of one pass")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+class Step:
+    """One CCD + impulse step on the GPU (inputs already on the device)."""
+
+    def __init__(self, ccc, torch, scene, iters, max_candidates):
+        dev = torch.device("cuda:0")
+        self.ccc, self.torch, self.iters, self.maxc = ccc, torch, iters, max_candidates
+        self.faces = torch.from_numpy(scene.faces).to(dev)
+        self.flat = self.faces.reshape(-1)
+        self.x0 = torch.from_numpy(scene.x0).to(dev)
+        self.x1 = torch.from_numpy(scene.x1).to(dev)
+        self.t0 = self.x0[self.faces].unsqueeze(0).contiguous()
+        self.masses = torch.from_numpy(scene.corner_masses()).to(dev).unsqueeze(0).contiguous()
+        self.nv = scene.x0.shape[0]
+        self.counts = []
+
+    def run(self):
+        torch, ccc = self.torch, self.ccc
+        x1 = self.x1
+        for _ in range(self.iters):
+            t1 = x1[self.faces].unsqueeze(0).contiguous()
+            bb, tree = ccc.bvh_motion(self.t0, t1)
+            dv, dx, cnt = ccc.compute_impulses(bb, tree, self.t0, t1, self.masses, self.maxc, 16)
+            self.counts.append(ccc.last_counts()[0])
+            d = dx[0].reshape(-1, 3)
+            ok = torch.isfinite(d).all(dim=1)
+            d = torch.where(ok[:, None], d, torch.zeros_like(d))
+            k = torch.where(ok, cnt[0].reshape(-1).to(x1.dtype), torch.zeros_like(ok, dtype=x1.dtype))
+            s = torch.zeros((self.nv, 3), dtype=x1.dtype, device=x1.device).index_add_(0, self.flat, d)
+            c = torch.zeros((self.nv,), dtype=x1.dtype, device=x1.device).index_add_(0, self.flat, k)
+            x1 = x1 + torch.where(c[:, None] > 0, s / c.clamp(min=1)[:, None], torch.zeros_like(s))
+        return x1
+
+
+def roofline(kt, counts, steps, iters):
+    """Roofline of the dominant kernel over the timed region."""
+    name, (ms, launches) = max(kt.items(), key=lambda kv: kv[1][0])
+    timed = counts[-steps * iters:]
+    m_total = float(sum(c[1] for c in timed))
+    k_total = float(sum(c[0] for c in timed))
+    avg_s = ms / 1e3 / max(launches, 1)
+    if name.startswith("ccd"):
+        work = FLOPS_PER_SUBPRIM * m_total / max(launches, 1)
+        achieved = work / avg_s / 1e12
+        return name, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                      "kernel": name, "avg_launch_ms": avg_s * 1e3,
+                      "work_per_launch": f"{FLOPS_PER_SUBPRIM} fp64 flop x {m_total / max(launches, 1):.4g} VF/EE pairs"}
+    # memory-bound kernels: unique bytes per launch
+    per = {"traverse": 64.0, "subprim": 8.0}
+    b = per.get(name, 0.0) * (k_total / max(launches, 1))
+    achieved = b / avg_s / 1e9
+    return name, {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                  "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_launch_ms": avg_s * 1e3}
+
+
+def cpu_baseline(scene, stride, iters):
+    from oracle import oracle as O
+    t0, t1 = scene.soups()
+    m = scene.corner_masses()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 16))
+    O.impulse_pass(t0[:2000], t1[:2000], m[:2000], cores, 1)   # load / warm the library
+    t = time.perf_counter()
+    _, _, _, counts = O.impulse_pass(t0, t1, m, cores, stride)
+    dt = time.perf_counter() - t
+    pass_s = dt * stride
+    return {"value": 1.0 / (iters * pass_s), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": (f"one impulse pass (bvh_motion + compute_impulses) over every {stride}-th query leaf "
+                       f"of the 130,050-triangle cloth: {dt:.2f} s, {counts['candidates']} candidate pairs, "
+                       f"{counts['collisions']} collisions; scaled x{stride} per pass and x{iters} passes per step")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the workload shards by simulation; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cccollisions_amd as ccc
+    from cccollisions_amd import scenes
+
+    scene = scenes.folded_cloth(args.grid, seed=2)
+    step = Step(ccc, torch, scene, args.iters, args.max_candidates)
+    for _ in range(args.warmup):
+        step.run()
+    torch.cuda.synchronize()
+    ccc.set_profiling(True)
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        step.run()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    kt = ccc.kernel_times()
+    ccc.set_profiling(False)
+
+    name, roof = roofline(kt, step.counts, args.steps, args.iters)
+    timed = step.counts[-args.steps * args.iters:]
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+        "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"folded cloth {args.grid}x{args.grid} vertices ({scene.num_triangles} triangles), fp64, "
+                         f"{args.iters} impulse iterations per step (bvh_motion + compute_impulses + vertex update)"),
+            "triangles": scene.num_triangles,
+            "impulse_iterations": args.iters,
+            "max_candidates_per_triangle": args.max_candidates,
+            "mean_candidates_per_iter": float(np.mean([c[0] for c in timed])),
+            "mean_subprims_per_iter": float(np.mean([c[1] for c in timed])),
+            "mean_roots_per_iter": float(np.mean([c[2] for c in timed])),
+            "mean_collisions_per_iter": float(np.mean([c[3] for c in timed])),
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items())},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_stride, args.iters)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -35,6 +35,8 @@ compute_impulses_partial = _ext.compute_impulses_partial
 compute_impulses_from_collisions = _ext.compute_impulses_from_collisions
 set_fixed_capacity = _ext.set_fixed_capacity
 last_counts = _ext.last_counts
+set_profiling = _ext.set_profiling
+kernel_times = _ext.kernel_times
 
 ENTRY_POINTS = (
     "bvh", "bvh_motion", "find_collisions", "find_triangle_edge_collisions",
@@ -44,4 +46,4 @@ ENTRY_POINTS = (
     "compute_impulses_from_collisions",
 )
 
-__all__ = list(ENTRY_POINTS) + ["set_fixed_capacity", "last_counts"]
+__all__ = list(ENTRY_POINTS) + ["set_fixed_capacity", "last_counts", "set_profiling", "kernel_times"]

@@ -281,6 +281,7 @@ template <typename T>
 static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, T *bboxes, int64_t *tree,
                         hipStream_t s) {
     Workspace ws(ctx, s);
+    KernelTimer tm(ctx, s);
     const int64_t total = B * n;
     const int nb = (int)((n + kSortTile - 1) / kSortTile);
     T *boxes = ws.get_as<T>(total * 6);
@@ -294,10 +295,13 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     if ((st = fill_u32(counters, B * (n > 1 ? n - 1 : 1), 0u, s)) != CCC_OK) return st;
     if ((st = fill_u32(err, 1, 0u, s)) != CCC_OK) return st;
 
+    tm.begin("bvh_leaf_boxes");
     if (t1) hipLaunchKernelGGL((k_leaf_boxes<T, true>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
     else hipLaunchKernelGGL((k_leaf_boxes<T, false>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
+    tm.end();
     CCC_LAUNCH_CHECK("k_leaf_boxes");
 
+    tm.begin("bvh_sort");
     for (int pass = 0; pass < 4; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(k_sort_hist, dim3(nb, B), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
@@ -307,11 +311,13 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
         uint32_t *tk = ka; ka = kb; kb = tk;
         int32_t *tv = va; va = vb; vb = tv;
     }
+    tm.end();
     if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree);
+        CCC_TIMED(tm, "bvh_karras", hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree));
         CCC_LAUNCH_CHECK("k_karras");
     }
-    hipLaunchKernelGGL((k_refit<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, boxes, va, n, counters, bboxes, tree, err);
+    CCC_TIMED(tm, "bvh_refit",
+              hipLaunchKernelGGL((k_refit<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, boxes, va, n, counters, bboxes, tree, err));
     CCC_LAUNCH_CHECK("k_refit");
     unsigned herr = 0;
     CCC_HIP_TRY(hipMemcpyAsync(&herr, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));

@@ -107,4 +107,36 @@ const char *ccc_last_error(void) { return ccc::g_last_error.c_str(); }
 
 int ccc_abi_version(void) { return CCC_ABI_VERSION; }
 
+ccc_status ccc_context_set_profiling(ccc_context *ctx, int on) {
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    std::lock_guard<std::mutex> lock(ctx->prof_mu);
+    ctx->profiling = on != 0;
+    ctx->prof.clear();
+    return CCC_OK;
+}
+
+ccc_status ccc_context_kernel_time(ccc_context *ctx, const char *kernel, double *total_ms, int64_t *launches) {
+    if (!ctx || !kernel || !total_ms || !launches) return ccc::bad_arg("NULL argument");
+    std::lock_guard<std::mutex> lock(ctx->prof_mu);
+    auto it = ctx->prof.find(kernel);
+    *total_ms = it == ctx->prof.end() ? 0.0 : it->second.first;
+    *launches = it == ctx->prof.end() ? 0 : it->second.second;
+    return CCC_OK;
+}
+
+int ccc_context_kernel_names(ccc_context *ctx, char *buf, size_t len) {
+    if (!ctx) return -1;
+    std::string all;
+    {
+        std::lock_guard<std::mutex> lock(ctx->prof_mu);
+        for (auto &kv : ctx->prof) { if (!all.empty()) all += ","; all += kv.first; }
+    }
+    if (buf && len > 0) {
+        size_t n = all.size() < len - 1 ? all.size() : len - 1;
+        memcpy(buf, all.data(), n);
+        buf[n] = 0;
+    }
+    return (int)all.size();
+}
+
 }  // extern "C"

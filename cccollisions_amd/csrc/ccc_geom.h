@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ccc_libm.h"
+
 namespace ccc {
 
 template <typename T> struct V3 { T x, y, z; };
@@ -278,7 +280,8 @@ template <typename T> __device__ __forceinline__ void add_root(T *slots, double 
     n++;
 }
 
-// collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes.
+// collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes, with the
+// portable cbrt / acos / sin / cos of ccc_libm.h.
 // slots start at -1; only roots inside the add_root window are stored.
 template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef, T *slots) {
     const double a = coef[0], b = coef[1], c = coef[2], d = coef[3];
@@ -314,19 +317,19 @@ template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef,
         // :564-575 triple root through a float k
         float k = (float)(d / a);
         double x1;
-        if (k >= 0) x1 = (double)(cbrtf(k) * -1);
-        else x1 = (double)cbrtf(-k);
+        if (k >= 0) x1 = (double)(pm_cbrtf(k) * -1);
+        else x1 = (double)pm_cbrtf(-k);
         add_root(slots, x1, n);
     } else if (h <= 0) {
         // :576-591 three real roots (trigonometric form)
         double i = sqrt(((g * g) / 4.0) - h);
-        double j = cbrt(i);
-        double k = acos(-(g / (2 * i)));
+        double j = pm_cbrt(i);
+        double k = pm_acos(-(g / (2 * i)));
         double L = j * -1;
-        double M = cos(k / 3.0);
-        double N = sqrt(3.0) * sin(k / 3.0);
+        double M = pm_cos(k / 3.0);
+        double N = sqrt(3.0) * pm_sin(k / 3.0);
         double P = (b / (3.0 * a)) * -1;
-        double x1 = 2 * j * cos(k / 3.0) - (b / (3.0 * a));
+        double x1 = 2 * j * pm_cos(k / 3.0) - (b / (3.0 * a));
         double x2 = L * (M + N) + P;
         double x3 = L * (M - N) + P;
         add_root(slots, x1, n);
@@ -335,9 +338,9 @@ template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef,
     } else {
         // :592-611 one real root (Cardano)
         double R = -(g / 2.0) + sqrt(h);
-        double S = R >= 0 ? cbrt(R) : -cbrt(-R);
+        double S = R >= 0 ? pm_cbrt(R) : -pm_cbrt(-R);
         double V = -(g / 2.0) - sqrt(h);
-        double U = V >= 0 ? cbrt(V) : -cbrt(-V);
+        double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
         double x1 = (S + U) - (b / (3.0 * a));
         add_root(slots, x1, n);
     }

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +18,10 @@ struct ccc_context {
     ccc_free_fn free_fn;
     void *user;
     int num_cus;
+    // opt-in kernel timing (HIP events on the call's stream), accumulated per kernel name
+    bool profiling = false;
+    std::mutex prof_mu;
+    std::map<std::string, std::pair<double, int64_t>> prof;
 };
 
 namespace ccc {
@@ -40,6 +46,55 @@ class Workspace {
     std::vector<void *> ptrs_;
     bool failed_ = false;
 };
+
+// Records HIP events around kernel launches when the context profiles; collect() runs
+// after the call's stream synchronisation and adds the elapsed times to the context.
+class KernelTimer {
+  public:
+    KernelTimer(ccc_context *ctx, hipStream_t s) : ctx_(ctx), s_(s), on_(ctx && ctx->profiling) {}
+    ~KernelTimer() { collect(); }
+    void begin(const char *name) {
+        if (!on_) return;
+        Rec r{name, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) { on_ = false; return; }
+        (void)hipEventRecord(r.a, s_);
+        recs_.push_back(r);
+    }
+    void end() {
+        if (!on_ || recs_.empty()) return;
+        (void)hipEventRecord(recs_.back().b, s_);
+    }
+    void collect() {
+        if (recs_.empty()) return;
+        (void)hipStreamSynchronize(s_);
+        std::lock_guard<std::mutex> lock(ctx_->prof_mu);
+        for (auto &r : recs_) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                auto &e = ctx_->prof[r.name];
+                e.first += ms;
+                e.second += 1;
+            }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs_.clear();
+    }
+
+  private:
+    struct Rec { const char *name; hipEvent_t a, b; };
+    ccc_context *ctx_;
+    hipStream_t s_;
+    bool on_;
+    std::vector<Rec> recs_;
+};
+
+#define CCC_TIMED(timer, name, launch) \
+    do {                               \
+        (timer).begin(name);           \
+        launch;                        \
+        (timer).end();                 \
+    } while (0)
 
 inline ccc_status hip_fail(hipError_t e, const char *what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));

@@ -344,9 +344,13 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
 // (:409-445), solveCubic (:511-613), flatten of roots in [0, 1] (:732-765, including
 // duplicates), filterCollisions (:640-684); then
 //   COLLECT: rows (x, y, type) + root, truncated at the output capacity (:1028-1063);
-//   IMPULSE: impulse of the collision scattered with atomics into dv / dx / cnt
-//            (collisions_impulses.cu:1312-1366).
+//   IMPULSE: the accepted collisions of the block are staged in LDS and, once at least a
+//            block's worth has gathered, every thread takes one and scatters its impulse
+//            with atomics into dv / dx / cnt (collisions_impulses.cu:1312-1366).
+// The tetrahedron is gathered again for the filter so that it is not live across the
+// cubic solver (fewer VGPRs, more waves per SIMD).
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
+constexpr int kImpStage = 4 * kThreads;   // a tile adds at most 3 * kThreads to < kThreads staged
 
 template <typename T>
 __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
@@ -373,6 +377,13 @@ __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T ro
     }
 }
 
+// Hides a pointer's provenance from the optimiser so that loads through it are not
+// merged with earlier loads of the same addresses.
+template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ t0_all, const T *__restrict__ t1_all,
@@ -380,6 +391,9 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   DevCounters *__restrict__ ctr_all) {
+    __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
+    __shared__ T s_root[MODE == MODE_IMPULSE ? kImpStage : 1];
+    __shared__ int s_fill;
     const int b = blockIdx.y;
     DevCounters *ctr = ctr_all + b;
     const unsigned long long mc = ctr->items;
@@ -387,26 +401,51 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
     const int64_t *items = items_all + (int64_t)b * capM;
     const T *t0 = t0_all + (int64_t)b * n * 9;
     const T *t1 = t1_all + (int64_t)b * n * 9;
+    const T *masses = MODE == MODE_IMPULSE ? masses_all + (int64_t)b * n * 3 : nullptr;
+    T *dv = MODE == MODE_IMPULSE ? dv_all + (int64_t)b * n * 9 : nullptr;
+    T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
+    int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
     unsigned long long nroots = 0, ncoll = 0;
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+
+    auto flush = [&](int f) {
+        for (int j = threadIdx.x; j < f; j += kThreads) {
+            int64_t x, y;
+            int type;
+            unpack_item(s_item[j], x, y, type);
+            const T *p0 = opaque_ptr(t0), *p1 = opaque_ptr(t1);
+            const MTet<T> mt = gather_mtet(p0, p1, x, y, type);
+            scatter_impulse(mt, type, s_root[j], x, y, masses, dv, dx, cnt);
+        }
+    };
+
     for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
         const int64_t i = tile * kThreads + threadIdx.x;
-        int64_t x = 0, y = 0;
+        int64_t x = 0, y = 0, it = 0;
         int type = 0;
         T acc[3];
         int nacc = 0;
-        MTet<T> m;
         if (i < M) {
-            unpack_item(items[i], x, y, type);
-            m = gather_mtet(t0, t1, x, y, type);
+            it = items[i];
+            unpack_item(it, x, y, type);
             T cf[4], sl[3];
-            cubic_coeffs(m, cf);
+            {
+                const MTet<T> m = gather_mtet(t0, t1, x, y, type);
+                cubic_coeffs(m, cf);
+            }
             solve_cubic(cf, sl);
-#pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                const T r = sl[s];
-                if (!(r >= 0 && r <= 1)) continue;
-                ++nroots;
-                if (filter_collision(m, type, r)) acc[nacc++] = r;
+            const bool any = (sl[0] >= 0 && sl[0] <= 1) || (sl[1] >= 0 && sl[1] <= 1) || (sl[2] >= 0 && sl[2] <= 1);
+            if (any) {
+                const T *p0 = opaque_ptr(t0), *p1 = opaque_ptr(t1);
+                const MTet<T> m = gather_mtet(p0, p1, x, y, type);
+#pragma unroll 1
+                for (int s = 0; s < 3; ++s) {
+                    const T r = sl[s];
+                    if (!(r >= 0 && r <= 1)) continue;
+                    ++nroots;
+                    if (filter_collision(m, type, r)) acc[nacc++] = r;
+                }
             }
         }
         if (MODE == MODE_COLLECT) {
@@ -420,13 +459,28 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
                 }
             }
         } else {
-            const T *masses = masses_all + (int64_t)b * n * 3;
-            T *dv = dv_all + (int64_t)b * n * 9;
-            T *dx = dx_all + (int64_t)b * n * 9;
-            int *cnt = cnt_all + (int64_t)b * n * 3;
-            for (int k = 0; k < nacc; ++k) scatter_impulse(m, type, acc[k], x, y, masses, dv, dx, cnt);
-            ncoll += (unsigned long long)nacc;
+            if (nacc) {
+                const int pos = atomicAdd(&s_fill, nacc);
+                for (int k = 0; k < nacc; ++k) {
+                    s_item[pos + k] = it;
+                    s_root[pos + k] = acc[k];
+                }
+            }
+            __syncthreads();
+            const int f = s_fill;
+            if (f >= kThreads) {
+                flush(f);
+                if (threadIdx.x == 0) ncoll += (unsigned long long)f;
+                __syncthreads();
+                if (threadIdx.x == 0) s_fill = 0;
+            }
+            __syncthreads();
         }
+    }
+    if (MODE == MODE_IMPULSE) {
+        const int f = s_fill;
+        flush(f);
+        if (threadIdx.x == 0) ncoll += (unsigned long long)f;
     }
     block_add(&ctr->roots, nroots);
     if (MODE == MODE_IMPULSE) block_add(&ctr->coll, ncoll);
@@ -565,6 +619,7 @@ static ccc_status run_query(const Query &q) {
     const int64_t B = q.B, n = q.n, rows = 2 * n - 1;
     const int64_t nint = n > 1 ? n - 1 : 1;
     Workspace ws(q.ctx, s);
+    KernelTimer tm(q.ctx, s);
     DevCounters *ctr = ws.get_as<DevCounters>(B);
     int *claims = ws.get_as<int>(B * n);
     int *climb = ws.get_as<int>(B * nint);
@@ -597,9 +652,11 @@ static ccc_status run_query(const Query &q) {
     if (st != CCC_OK) return st;
 
     const T *bb = (const T *)q.bboxes;
+    tm.begin("tree_prep");
     hipLaunchKernelGGL(k_check_tree, dim3(grid_for(rows, 256), B), dim3(256), 0, s, q.tree, n, claims, ctr);
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
+    tm.end();
     CCC_LAUNCH_CHECK("tree preparation");
 
     // traversal, re-run once with the exact size if the capacity was exceeded
@@ -610,12 +667,14 @@ static ccc_status run_query(const Query &q) {
     for (int attempt = 0; attempt < 2; ++attempt) {
         pairs = ws.get_as<int2>(B * capK);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        tm.begin("traverse");
         if (q.to_check)
             hipLaunchKernelGGL((k_traverse<T, true>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
                                nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
         else
             hipLaunchKernelGGL((k_traverse<T, false>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
                                nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
+        tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
         CCC_HIP_TRY(hipStreamSynchronize(s));
@@ -641,25 +700,28 @@ static ccc_status run_query(const Query &q) {
     const T thr = (T)q.thr;
     const unsigned gK = persistent_grid(q.ctx, kmax, B);
     if (q.narrow == Narrow::SAT) {
-        hipLaunchKernelGGL((k_sat<T>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n, thr, q.out_rows,
-                           q.out_cap, ctr);
+        CCC_TIMED(tm, "sat", hipLaunchKernelGGL((k_sat<T>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n, thr,
+                                                q.out_rows, q.out_cap, ctr));
         CCC_LAUNCH_CHECK("k_sat");
     } else if (q.narrow == Narrow::MINCOL) {
-        hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0,
-                           (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr);
+        CCC_TIMED(tm, "mincol", hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0,
+                                                   (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr));
         CCC_LAUNCH_CHECK("k_subprim");
     } else {
         const int64_t capM = std::max<int64_t>(15 * kmax, 1);
         int64_t *items = ws.get_as<int64_t>(B * capM);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        tm.begin("subprim");
         if (q.narrow == Narrow::PROXIMITY)
             hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)nullptr,
                                n, thr, items, capM, ctr);
         else
             hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)q.t1,
                                n, thr, items, capM, ctr);
+        tm.end();
         CCC_LAUNCH_CHECK("k_subprim");
         const unsigned gM = persistent_grid(q.ctx, capM, B);
+        tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
         if (q.narrow == Narrow::PROXIMITY)
             hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr, q.out_rows,
                                q.out_cap, ctr);
@@ -671,6 +733,7 @@ static ccc_status run_query(const Query &q) {
             hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
                                (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0, (T *)q.dv,
                                (T *)q.dx, q.cnt, ctr);
+        tm.end();
         CCC_LAUNCH_CHECK("narrow phase");
     }
     CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
@@ -721,9 +784,12 @@ static ccc_status from_collisions(ccc_context *ctx, const int64_t *coll, const v
     else { st = fill_u32(dv, B * n * 9, 0u, s); if (st == CCC_OK) st = fill_u32(dx, B * n * 9, 0u, s); }
     if (st == CCC_OK) st = fill_u32(cnt, B * n * 3, 0u, s);
     if (st != CCC_OK) return st;
+    KernelTimer tm(ctx, s);
     if (m > 0) {
+        tm.begin("from_collisions");
         hipLaunchKernelGGL((k_from_collisions<T>), dim3(grid_for(m, kThreads), B), dim3(kThreads), 0, s, coll, (const T *)roots, m,
                            (const T *)t0, (const T *)t1, (const T *)masses, n, (T *)dv, (T *)dx, cnt);
+        tm.end();
         CCC_LAUNCH_CHECK("k_from_collisions");
     }
     CCC_HIP_TRY(hipStreamSynchronize(s));

@@ -9,7 +9,9 @@
 #include <c10/hip/HIPStream.h>
 #include <torch/extension.h>
 
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/cccollisions_amd.h"
@@ -75,18 +77,21 @@ void check_soup(const at::Tensor &t, int64_t B, int64_t n, const char *name) {
 thread_local uint32_t g_flags = CCC_FLAG_NONE;
 thread_local std::vector<ccc_counts> g_counts;
 
+// Overflow warnings become Python UserWarnings (the reference printed them to stdout).
+void warn(const char *msg) {
+    if (PyErr_WarnEx(PyExc_UserWarning, msg, 1) != 0) throw py::error_already_set();
+}
+
 void finish(ccc_status st, const char *what) {
     TORCH_CHECK((st & CCC_ERR_MASK) == CCC_OK, "cccollisions.", what, " failed: ", ccc_status_string(st), ": ", ccc_last_error());
     if (st & CCC_WARN_CANDIDATES) {
-        if (g_flags & CCC_FLAG_FIXED_CAPACITY) {
-            TORCH_WARN("Number of candidate collisions exceeds maximal allowed number. Some of the candidate collisions are omitted.");
-        } else {
-            TORCH_WARN("Number of candidate collisions exceeds max_candidates_per_triangle * N; the candidate buffer was grown to hold all of them.");
-        }
+        if (g_flags & CCC_FLAG_FIXED_CAPACITY)
+            warn("Number of candidate collisions exceeds maximal allowed number. Some of the candidate collisions are omitted.");
+        else
+            warn("Number of candidate collisions exceeds max_candidates_per_triangle * N; the candidate buffer was grown to hold all of them.");
     }
-    if (st & CCC_WARN_TRUNCATED) {
-        TORCH_WARN("Number of actual collisions exceeds maximal allowed number. Some of the collisions are omitted.");
-    }
+    if (st & CCC_WARN_TRUNCATED)
+        warn("Number of actual collisions exceeds maximal allowed number. Some of the collisions are omitted.");
 }
 
 ccc_counts *counts_buffer(int64_t B) {
@@ -338,6 +343,36 @@ std::vector<torch::Tensor> compute_impulses_from_collisions(at::Tensor collision
 // ---- extras (not in the reference) -------------------------------------------------------
 void set_fixed_capacity(bool on) { g_flags = on ? (g_flags | CCC_FLAG_FIXED_CAPACITY) : (g_flags & ~CCC_FLAG_FIXED_CAPACITY); }
 
+// Opt-in per-kernel timing with HIP events (device of the current stream).
+void set_profiling(bool on) {
+    const int dev = c10::hip::current_device();
+    auto t = at::empty({0}, at::TensorOptions().device(at::Device(at::kCUDA, dev)));
+    ccc_status st = ccc_context_set_profiling(context_for(t), on ? 1 : 0);
+    TORCH_CHECK(st == CCC_OK, "set_profiling failed");
+}
+
+std::map<std::string, std::pair<double, int64_t>> kernel_times() {
+    const int dev = c10::hip::current_device();
+    auto t = at::empty({0}, at::TensorOptions().device(at::Device(at::kCUDA, dev)));
+    ccc_context *c = context_for(t);
+    std::vector<char> buf((size_t)std::max(ccc_context_kernel_names(c, nullptr, 0), 0) + 1);
+    ccc_context_kernel_names(c, buf.data(), buf.size());
+    std::map<std::string, std::pair<double, int64_t>> out;
+    std::string names(buf.data());
+    size_t pos = 0;
+    while (pos < names.size()) {
+        size_t e = names.find(',', pos);
+        if (e == std::string::npos) e = names.size();
+        std::string k = names.substr(pos, e - pos);
+        double ms = 0;
+        int64_t n = 0;
+        ccc_context_kernel_time(c, k.c_str(), &ms, &n);
+        out[k] = {ms, n};
+        pos = e + 1;
+    }
+    return out;
+}
+
 std::vector<std::vector<int64_t>> last_counts() {
     std::vector<std::vector<int64_t>> out;
     for (const auto &c : g_counts) out.push_back({c.candidates, c.subprims, c.roots, c.collisions});
@@ -377,5 +412,7 @@ PYBIND11_MODULE(_cccollisions, m) {
           py::arg("triangles"), py::arg("triangles_next"), py::arg("triangles_mass"));
     m.def("set_fixed_capacity", &set_fixed_capacity,
           "Keep the reference candidate capacity (max_candidates_per_triangle * N) instead of growing it", py::arg("on"));
+    m.def("set_profiling", &set_profiling, "Time every kernel launch with HIP events (clears totals)", py::arg("on"));
+    m.def("kernel_times", &kernel_times, "Per-kernel {name: (total_ms, launches)} since profiling was enabled");
     m.def("last_counts", &last_counts, "Per-batch [K, M, V, R] stage counts of the last query on this thread");
 }

@@ -88,12 +88,15 @@ def batched_cloths(count: int = 64, lo: int = 2000, hi: int = 200000, seed: int 
 
 def apply_corrections(faces: np.ndarray, x1: np.ndarray, dx: np.ndarray, cnt: np.ndarray) -> np.ndarray:
     """Caller-side update after one impulse pass: every vertex moves by the mean of the
-    position corrections its triangle corners received (corners with cnt = 0 add
-    nothing).  Returns the new end-of-step positions."""
+    position corrections its triangle corners received.  A corner whose summed correction
+    is not finite (the reference accepts collisions with NaN parameters, whose normal is
+    NaN) contributes neither its correction nor its count.  Returns the new end-of-step
+    positions."""
+    ok = np.isfinite(dx.reshape(-1, 3)).all(axis=1)
     vs = np.zeros_like(x1)
     vc = np.zeros(x1.shape[0])
-    np.add.at(vs, faces.reshape(-1), dx.reshape(-1, 3))
-    np.add.at(vc, faces.reshape(-1), cnt.reshape(-1).astype(np.float64))
+    np.add.at(vs, faces.reshape(-1)[ok], dx.reshape(-1, 3)[ok])
+    np.add.at(vc, faces.reshape(-1)[ok], cnt.reshape(-1)[ok].astype(np.float64))
     out = x1.copy()
     hit = vc > 0
     out[hit] += vs[hit] / vc[hit, None]

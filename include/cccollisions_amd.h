@@ -78,6 +78,14 @@ const char *ccc_status_string(ccc_status status);
 const char *ccc_last_error(void);
 int ccc_abi_version(void);
 
+/* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,
+ * every kernel launch of a call is bracketed by HIP events on the call's stream and the
+ * elapsed times are accumulated per kernel name.  Enabling or disabling clears the totals. */
+ccc_status ccc_context_set_profiling(ccc_context *ctx, int on);
+ccc_status ccc_context_kernel_time(ccc_context *ctx, const char *kernel, double *total_ms, int64_t *launches);
+/* Comma-separated names of the kernels timed so far; returns the full length. */
+int ccc_context_kernel_names(ccc_context *ctx, char *buf, size_t len);
+
 /* Replaces build_bvh(triangles, ...) -- src/build_bvh.cu:381-449 (binding
  * src/cccollisions.cpp:102-116).  Static leaf boxes. */
 ccc_status ccc_bvh(ccc_context *ctx, const void *triangles, int64_t batch, int64_t n, int32_t dtype,

@@ -4,6 +4,7 @@
  * exactly as written (explicit fma() calls model nvcc's contraction where noted).
  */
 #include "ccc_oracle.h"
+#include "ccc_libm_ref.h"
 
 #include <float.h>
 #include <math.h>
@@ -197,12 +198,14 @@ double orc_dd_cube_lead(double b) {
 
 /* ---- CPU baseline: one CCD + impulse pass ------------------------------------------
  * bvh_motion (orc_bvh_f64) -> traversal -> narrow phase -> impulses, parallelised with
- * OpenMP over query leaves and over candidate pairs.  Impulse sums use per-thread
+ * OpenMP over query leaves.  query_stride > 1 processes only every query_stride-th query
+ * leaf (a bounded sample of the pass for timing).  Impulse sums use per-thread
  * partial arrays reduced at the end (sum order differs from the sequential oracle at the
  * ULP level only). */
 int orc_impulse_pass_f64(const double *tris, const double *tris_next, const double *masses,
-                         int64_t n, int nthreads, double *dv, double *dx, int32_t *cnt,
-                         orc_counts *counts) {
+                         int64_t n, int nthreads, int64_t query_stride, double *dv, double *dx,
+                         int32_t *cnt, orc_counts *counts) {
+    if (query_stride < 1) query_stride = 1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -236,7 +239,7 @@ int orc_impulse_pass_f64(const double *tris, const double *tris_next, const doub
         pcnt[tid] = (int32_t *)calloc(3 * (size_t)n, sizeof(int32_t));
         int64_t stk[64];
 #pragma omp for schedule(dynamic, 64)
-        for (int64_t p = 0; p < n - 1; ++p) {
+        for (int64_t p = 0; p < n - 1; p += query_stride) {
             int64_t qrow = n - 1 + p;
             box_f64 q = load_box_f64(bb + 6 * qrow);
             int64_t qid = tree[4 * qrow + 3];
@@ -290,4 +293,19 @@ int orc_impulse_pass_f64(const double *tris, const double *tris_next, const doub
     free(pdv); free(pdx); free(pcnt); free(rm); free(bb); free(tree);
     if (counts) { counts->candidates = K; counts->subprims = M; counts->roots = V; counts->collisions = R; }
     return 0;
+}
+
+/* Vector evaluation of the portable math functions (ccc_libm_ref.h) for the pinning tests:
+ * fn 0 cbrt, 1 acos, 2 sin, 3 cos, 4 cbrtf (float argument and result). */
+void orc_libm_eval(int fn, const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: y[i] = orc_cbrt(x[i]); break;
+            case 1: y[i] = orc_acos(x[i]); break;
+            case 2: y[i] = orc_sin(x[i]); break;
+            case 3: y[i] = orc_cos(x[i]); break;
+            case 4: y[i] = (double)orc_cbrtf((float)x[i]); break;
+            default: y[i] = NAN;
+        }
+    }
 }

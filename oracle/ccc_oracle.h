@@ -79,12 +79,13 @@ int orc_karras_node(const uint32_t *codes, const int32_t *ids, int64_t n, int64_
 int orc_karras_node_literal(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i,
                             int64_t *left, int64_t *right);
 uint32_t orc_morton3d_f64(double x, double y, double z);
+void orc_libm_eval(int fn, const double *x, double *y, int64_t n);
 
 /* Whole CCD+impulse pass (bvh_motion -> traverse -> narrow phase -> impulses) used as
  * the CPU baseline; OpenMP over queries and candidate pairs.  Returns 0 on success. */
 int orc_impulse_pass_f64(const double *tris, const double *tris_next, const double *masses,
-                         int64_t n, int nthreads, double *dv, double *dx, int32_t *cnt,
-                         orc_counts *counts);
+                         int64_t n, int nthreads, int64_t query_stride, double *dv, double *dx,
+                         int32_t *cnt, orc_counts *counts);
 
 #ifdef __cplusplus
 }

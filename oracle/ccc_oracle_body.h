@@ -271,7 +271,8 @@ static inline void FN(add_root)(REAL *slots, double root, int *n) {
     (*n)++;
 }
 
-/* collisions_continuous.cu:511-613 solveCubic (always double precision). */
+/* collisions_continuous.cu:511-613 solveCubic (always double precision); cbrt/acos/sin/cos are
+ * the portable versions of ccc_libm_ref.h. */
 static void FN(solve_cubic)(const REAL *coef, REAL *slots) {
     double a = coef[0], b = coef[1], c = coef[2], d = coef[3];
     int n = 0;
@@ -305,19 +306,19 @@ static void FN(solve_cubic)(const REAL *coef, REAL *slots) {
             /* :564-575 triple root through a float k */
             float k = (float)(d / a);
             double x1;
-            if (k >= 0) x1 = (double)(cbrtf(k) * -1);
-            else x1 = (double)cbrtf(-k);
+            if (k >= 0) x1 = (double)(orc_cbrtf(k) * -1);
+            else x1 = (double)orc_cbrtf(-k);
             FN(add_root)(slots, x1, &n);
         } else if (h <= 0) {
             /* :576-591 three real roots, trigonometric form */
             double i = sqrt(((g * g) / 4.0) - h);
-            double j = cbrt(i);
-            double k = acos(-(g / (2 * i)));
+            double j = orc_cbrt(i);
+            double k = orc_acos(-(g / (2 * i)));
             double L = j * -1;
-            double M = cos(k / 3.0);
-            double N = sqrt(3.0) * sin(k / 3.0);
+            double M = orc_cos(k / 3.0);
+            double N = sqrt(3.0) * orc_sin(k / 3.0);
             double P = (b / (3.0 * a)) * -1;
-            double x1 = 2 * j * cos(k / 3.0) - (b / (3.0 * a));
+            double x1 = 2 * j * orc_cos(k / 3.0) - (b / (3.0 * a));
             double x2 = L * (M + N) + P;
             double x3 = L * (M - N) + P;
             FN(add_root)(slots, x1, &n);
@@ -327,9 +328,9 @@ static void FN(solve_cubic)(const REAL *coef, REAL *slots) {
             /* :592-611 one real root, Cardano */
             double R = -(g / 2.0) + sqrt(h);
             double S, U;
-            if (R >= 0) S = cbrt(R); else S = -cbrt(-R);
+            if (R >= 0) S = orc_cbrt(R); else S = -orc_cbrt(-R);
             double V = -(g / 2.0) - sqrt(h);
-            if (V >= 0) U = cbrt(V); else U = -cbrt(-V);
+            if (V >= 0) U = orc_cbrt(V); else U = -orc_cbrt(-V);
             double x1 = (S + U) - (b / (3.0 * a));
             FN(add_root)(slots, x1, &n);
         }

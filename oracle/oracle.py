@@ -79,7 +79,8 @@ def _declare(L):
         getattr(L, f).restype = C.c_int
     L.orc_morton3d_f64.argtypes = [C.c_double, C.c_double, C.c_double]
     L.orc_morton3d_f64.restype = C.c_uint32
-    L.orc_impulse_pass_f64.argtypes = [_P, _P, _P, _I64, C.c_int, _P, _P, _P, C.POINTER(Counts)]
+    L.orc_libm_eval.argtypes = [C.c_int, _P, _P, _I64]
+    L.orc_impulse_pass_f64.argtypes = [_P, _P, _P, _I64, C.c_int, _I64, _P, _P, _P, C.POINTER(Counts)]
     L.orc_impulse_pass_f64.restype = C.c_int
 
 
@@ -270,8 +271,20 @@ def morton3d(x, y, z):
     return lib().orc_morton3d_f64(x, y, z)
 
 
-def impulse_pass(tris, tris_next, masses, nthreads=0):
-    """CPU baseline: bvh_motion + compute_impulses for one mesh (OpenMP)."""
+LIBM_FUNCS = {"cbrt": 0, "acos": 1, "sin": 2, "cos": 3, "cbrtf": 4}
+
+
+def libm(name, x):
+    """Portable cbrt/acos/sin/cos used by the oracle's cubic solver (ccc_libm_ref.h)."""
+    x = np.ascontiguousarray(np.asarray(x, np.float64).ravel())
+    y = np.empty_like(x)
+    lib().orc_libm_eval(LIBM_FUNCS[name], _ptr(x), _ptr(y), x.size)
+    return y
+
+
+def impulse_pass(tris, tris_next, masses, nthreads=0, query_stride=1):
+    """CPU baseline: bvh_motion + compute_impulses for one mesh (OpenMP over query leaves;
+    query_stride > 1 runs only every query_stride-th query leaf)."""
     tris = _c(tris, np.float64)
     tris_next = _c(tris_next, np.float64)
     masses = _c(masses, np.float64)
@@ -281,7 +294,7 @@ def impulse_pass(tris, tris_next, masses, nthreads=0):
     cnt = np.zeros((n, 3), np.int32)
     c = Counts()
     rc = lib().orc_impulse_pass_f64(_ptr(tris), _ptr(tris_next), _ptr(masses), n, nthreads,
-                                    _ptr(dv), _ptr(dx), _ptr(cnt), C.byref(c))
+                                    query_stride, _ptr(dv), _ptr(dx), _ptr(cnt), C.byref(c))
     if rc != 0:
         raise RuntimeError(f"orc_impulse_pass failed: {rc}")
     return dv, dx, cnt, c.as_dict()

@@ -1,6 +1,7 @@
 """Parity of the HIP path (through the torch extension, which calls the C ABI) against the
-CPU oracle on seeded inputs.  Index / pair / collision lists are compared bit-exactly as
-sorted multisets; roots and impulses within 1e-9 relative."""
+CPU oracle on seeded inputs.  Trees, boxes, pair and collision lists and roots are compared
+bit-exactly (lists as sorted multisets); impulse sums, accumulated with fp64 atomics in a
+different order, within 1e-9 relative."""
 import numpy as np
 import pytest
 
@@ -85,7 +86,8 @@ def _compare_collisions(gc, gr, coll, roots):
     ia = np.lexsort((gr, g[:, 2], g[:, 1], g[:, 0]))
     ib = np.lexsort((roots, coll[:, 2], coll[:, 1], coll[:, 0]))
     np.testing.assert_array_equal(ka[ia][:, :3].astype(np.int64), kb[ib][:, :3].astype(np.int64))
-    _check_close(ka[ia][:, 3], kb[ib][:, 3], "roots")
+    # the oracle and the kernels share the portable cbrt/acos/sin/cos: roots agree bit for bit
+    np.testing.assert_array_equal(ka[ia][:, 3], kb[ib][:, 3])
 
 
 @pytest.mark.parametrize("name", list(SCENES))
