@@ -1,0 +1,256 @@
+"""Pinning the oracle's queries: traversal vs an independent O(N^2) enumeration, the 15-way
+sub-primitive filter, proximity and the triangle-triangle SAT vs numpy restatements.
+CPU only."""
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.conftest import sorted_rows
+
+EDGES = [(0, 1), (0, 2), (1, 2)]
+
+
+def np_pairs(bb, tree, thr):
+    """All pairs of sorted leaf positions p < q whose boxes overlap with p's box as the
+    query (aabb.cu:101-111), as (min id, max id)."""
+    n = (bb.shape[0] + 1) // 2
+    lb = bb[n - 1:]
+    ids = tree[n - 1:, 3]
+    thr = lb.dtype.type(thr)
+    p, q = np.triu_indices(n, 1)
+    a, b = lb[p], lb[q]
+    ok = np.ones(p.shape, bool)
+    for k in range(3):
+        ok &= (a[:, k] <= (b[:, 3 + k] + thr)) & (a[:, 3 + k] >= (b[:, k] - thr))
+    x, y = ids[p[ok]], ids[q[ok]]
+    return np.stack([np.minimum(x, y), np.maximum(x, y)], 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("thr", [0.0, 1e-3, 3e-2])
+@pytest.mark.parametrize("motion", [False, True])
+def test_traversal_equals_bruteforce(oracle, dtype, thr, motion):
+    t0, t1, _ = cases.random_soup(400, seed=2, dtype=dtype)
+    bb, tree, _, _ = oracle.bvh(t0, t1 if motion else None)
+    pairs, total = oracle.traverse(bb, tree, thr)
+    assert total == pairs.shape[0]
+    ref = np_pairs(bb, tree, thr)
+    np.testing.assert_array_equal(sorted_rows(pairs), sorted_rows(ref))
+    np.testing.assert_array_equal(sorted_rows(oracle.bruteforce_pairs(bb, tree, thr)), sorted_rows(ref))
+
+
+def test_traversal_topology_independent(oracle):
+    """The candidate set does not depend on the tree: a fitted-Morton tree over the same
+    boxes gives the same pairs (threshold 0, where the overlap test is symmetric)."""
+    t0, t1, _ = cases.random_soup(600, seed=8)
+    bb0, tr0, _, _ = oracle.bvh(t0, t1, morton_mode=0)
+    bb1, tr1, _, _ = oracle.bvh(t0, t1, morton_mode=1)
+    assert not np.array_equal(tr0, tr1)
+    np.testing.assert_array_equal(sorted_rows(oracle.traverse(bb0, tr0)[0]), sorted_rows(oracle.traverse(bb1, tr1)[0]))
+
+
+def test_traversal_capacity(oracle):
+    t0, t1, _ = cases.random_soup(300, seed=1)
+    bb, tree, _, _ = oracle.bvh(t0, t1)
+    full, total = oracle.traverse(bb, tree)
+    part, total2 = oracle.traverse(bb, tree, cap=10)
+    assert total2 == total and part.shape[0] == 10
+
+
+def test_partial_traversal(oracle):
+    """collisions_continuous.cu:127-230: ordering off; a to_check query reports a leaf c when
+    c is not to_check or query id < c id."""
+    t0, t1, _ = cases.random_soup(300, seed=4)
+    bb, tree, _, _ = oracle.bvh(t0, t1)
+    n = 300
+    mask = np.random.default_rng(3).random(n) < 0.25
+    got = oracle.traverse_partial(bb, tree, mask)
+    lb = bb[n - 1:]
+    ids = tree[n - 1:, 3]
+    rows = []
+    for p in range(n):
+        if not mask[p]:
+            continue
+        for q in range(n):
+            a, b = lb[p], lb[q]
+            if not all(a[k] <= b[3 + k] and a[3 + k] >= b[k] for k in range(3)):
+                continue
+            qi, ci = ids[p], ids[q]
+            if (not mask[ci]) or qi < ci:
+                rows.append((min(qi, ci), max(qi, ci)))
+    np.testing.assert_array_equal(sorted_rows(got), sorted_rows(np.array(rows).reshape(-1, 2)))
+
+
+def np_swept_mask(t0, t1, x, y):
+    """15-bit sub-primitive mask (collisions_continuous.cu:234-318) in numpy."""
+    a0, a1, b0, b1 = t0[x], t1[x], t0[y], t1[y]
+    if any((a0[i] == b0[j]).all() for i in range(3) for j in range(3)):
+        return 0
+    va = [(np.minimum(a0[i], a0[i] + (a1[i] - a0[i])), np.maximum(a0[i], a0[i] + (a1[i] - a0[i]))) for i in range(3)]
+    vb = [(np.minimum(b0[i], b0[i] + (b1[i] - b0[i])), np.maximum(b0[i], b0[i] + (b1[i] - b0[i]))) for i in range(3)]
+
+    def uni(*bs):
+        return np.min([b[0] for b in bs], 0), np.max([b[1] for b in bs], 0)
+
+    def ov(p, q):
+        return bool(((p[0] <= q[1]) & (p[1] >= q[0])).all())
+
+    ea = [uni(va[i], va[j]) for i, j in EDGES]
+    eb = [uni(vb[i], vb[j]) for i, j in EDGES]
+    fa, fb = uni(*va), uni(*vb)
+    m = 0
+    for t in range(9):
+        m |= ov(ea[t // 3], eb[t % 3]) << t
+    for k in range(3):
+        m |= ov(fa, vb[k]) << (9 + k)
+        m |= ov(va[k], fb) << (12 + k)
+    return m
+
+
+def test_mincol_filter(oracle):
+    t0, t1, _ = cases.random_soup(200, seed=6)
+    bb, tree, _, _ = oracle.bvh(t0, t1)
+    pairs, _ = oracle.traverse(bb, tree)
+    got = oracle.mincol(t0, t1, pairs)
+    rows = [(x, y, t) for x, y in pairs for t in range(15) if (np_swept_mask(t0, t1, x, y) >> t) & 1]
+    np.testing.assert_array_equal(sorted_rows(got), sorted_rows(np.array(rows)))
+
+
+def test_shared_vertex_pairs_rejected(oracle):
+    t0, t1, _ = cases.shared_vertex_soup(150)
+    bb, tree, _, _ = oracle.bvh(t0, t1)
+    pairs, _ = oracle.traverse(bb, tree)
+    got = oracle.mincol(t0, t1, pairs)
+    shared = {(x, y) for x, y in pairs if any((t0[x][i] == t0[y][j]).all() for i in range(3) for j in range(3))}
+    assert shared, "scene must contain adjacent pairs"
+    assert not any((x, y) in shared for x, y, _ in got)
+
+
+# ---- proximity -------------------------------------------------------------------------
+
+def np_dist_ee(v0, v1, v2, v3):
+    da, db, dc = v1 - v0, v3 - v2, v2 - v0
+    cab = np.cross(da, db)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.dot(np.cross(dc, db), cab) / np.dot(cab, cab)
+        t = np.dot(np.cross(dc, da), cab) / np.dot(cab, cab)
+    if np.isnan(s) or np.isnan(t):
+        best = [np.float32(np.finfo(np.float32).max), -1.0, -1.0]
+        for p, a, b, set_s, sec in ((v0, v2, v3, False, 0.0), (v1, v2, v3, False, 1.0),
+                                    (v2, v0, v1, True, 0.0), (v3, v0, v1, True, 1.0)):
+            ab, ap = b - a, p - a
+            with np.errstate(invalid="ignore", divide="ignore"):
+                tt = np.dot(ap, ab) / np.dot(ab, ab)
+            tt = 0.0 if tt < 0 else (1.0 if tt > 1 else tt)
+            c = a + tt * ab
+            nd = np.sqrt(np.float32(np.dot(p - c, p - c)))
+            if nd < best[0]:
+                best = [nd, tt, sec] if set_s else [nd, sec, tt]
+        s, t = best[1], best[2]
+    if not (0 <= s <= 1 and 0 <= t <= 1):
+        return -1.0
+    d = (v0 + da * s) - (v2 + db * t)
+    return float(np.sqrt(np.float32(np.dot(d, d))))
+
+
+def np_dist_vf(v0, v1, v2, vt):
+    e1, e2, ep = v0 - v1, v2 - v1, vt - v1
+    d00, d01, d11, d20, d21 = e1 @ e1, e1 @ e2, e2 @ e2, ep @ e1, ep @ e2
+    den = d00 * d11 - d01 * d01
+    u = (d11 * d20 - d01 * d21) / den
+    w = (d00 * d21 - d01 * d20) / den
+    v = 1 - w - u
+    if not (0 <= v <= 1 and 0 <= w <= 1 and 0 <= u <= 1):
+        return -1.0
+    d = vt - (v0 * u + v1 * v + v2 * w)
+    return float(np.sqrt(np.float32(np.dot(d, d))))
+
+
+def test_proximity_numpy(oracle):
+    tri = cases.proximity_fold(12, gap=1e-3)
+    thr = float(np.float32(2e-3))
+    bb, tree, _, _ = oracle.bvh(tri)
+    rows, counts = oracle.find_proximity(bb, tree, tri, 2e-3)
+    pairs, _ = oracle.traverse(bb, tree, thr)
+    ref = []
+    for x, y in pairs:
+        a, b = tri[x], tri[y]
+        if any((a[i] == b[j]).all() for i in range(3) for j in range(3)):
+            continue
+        for t in range(15):
+            if t < 9:
+                i0, i1 = EDGES[t // 3]
+                j0, j1 = EDGES[t % 3]
+                pa = np.array([a[i0], a[i1]]); pb = np.array([b[j0], b[j1]])
+                boxa = (pa.min(0), pa.max(0)); boxb = (pb.min(0), pb.max(0))
+                pts = (a[i0], a[i1], b[j0], b[j1])
+            elif t < 12:
+                boxa = (a.min(0), a.max(0)); boxb = (b[t - 9], b[t - 9])
+                pts = (a[0], a[1], a[2], b[t - 9])
+            else:
+                boxa = (a[t - 12], a[t - 12]); boxb = (b.min(0), b.max(0))
+                pts = (a[t - 12], b[0], b[1], b[2])
+            if not ((boxa[0] <= boxb[1] + thr) & (boxa[1] >= boxb[0] - thr)).all():
+                continue
+            if t < 9:
+                d = np_dist_ee(*pts)
+            elif t < 12:
+                d = np_dist_vf(*pts)
+            else:
+                d = np_dist_vf(pts[1], pts[2], pts[3], pts[0])
+            if 0 <= d <= thr:
+                ref.append((x, y, t))
+    assert len(ref) > 0
+    np.testing.assert_array_equal(sorted_rows(rows), sorted_rows(np.array(ref)))
+
+
+def test_proximity_config1_counts(oracle):
+    """64x64 sine cloth, threshold 2e-3: no non-adjacent primitives are that close."""
+    from cccollisions_amd import scenes
+    v, f = scenes.proximity_cloth(64, seed=1)
+    tri = v[f]
+    bb, tree, _, _ = oracle.bvh(tri)
+    rows, counts = oracle.find_proximity(bb, tree, tri, 2e-3)
+    assert tri.shape[0] == 7938
+    assert counts["candidates"] == 64203 and counts["subprims"] == 65352 and rows.shape[0] == 0
+
+
+# ---- static triangle-triangle SAT --------------------------------------------------------
+
+def np_sat(a, b, thr):
+    ae = [a[1] - a[0], a[2] - a[0], a[2] - a[1]]
+    be = [b[1] - b[0], b[2] - b[0], b[2] - b[1]]
+    an, bn = np.cross(ae[1], ae[2]), np.cross(be[1], be[2])
+    axes = [an, bn] + [np.cross(i, j) for i in ae for j in be] + [np.cross(an, e) for e in ae] + [np.cross(an, e) for e in be]
+    for ax in axes:
+        pa, pb = a @ ax, b @ ax
+        if not (pa.min() <= pb.max() + thr and pa.max() >= pb.min() - thr):
+            return False
+    return True
+
+
+def test_sat_cases(oracle):
+    tri = np.array([
+        [[0, 0, 0], [1, 0, 0], [0, 1, 0]],           # 0: in z = 0
+        [[0.2, 0.2, -1], [0.2, 0.2, 1], [1, 1, 0]],  # 1: pierces 0
+        [[0, 0, 2], [1, 0, 2], [0, 1, 2]],           # 2: parallel to 0, 2 apart
+        [[1, 0, 0], [2, 0, 0], [1, 1, 0]],           # 3: shares vertex (1,0,0) with 0
+        [[0.3, 0.3, 0.0005], [0.9, 0.3, 0.0005], [0.3, 0.9, 0.0005]],  # 4: 5e-4 above 0
+    ], np.float64)
+    pairs = np.array([[0, 1], [0, 2], [0, 3], [0, 4]])
+    got = {tuple(r) for r in oracle.static_collisions(tri, pairs, 0.0)}
+    assert got == {(0, 1)}
+    got = {tuple(r) for r in oracle.static_collisions(tri, pairs, 1e-3)}
+    assert got == {(0, 1), (0, 4)}          # 4 is within the threshold; 3 shares a vertex
+
+
+def test_sat_numpy(oracle):
+    t0, _, _ = cases.random_soup(300, seed=4)
+    bb, tree, _, _ = oracle.bvh(t0)
+    for thr in (0.0, 1e-3):
+        thr32 = float(np.float32(thr))
+        pairs, _ = oracle.traverse(bb, tree, thr32)
+        got = oracle.static_collisions(t0, pairs, thr32)
+        ref = [(x, y) for x, y in pairs
+               if np_sat(t0[x], t0[y], thr32) and not any((t0[x][i] == t0[y][j]).all() for i in range(3) for j in range(3))]
+        np.testing.assert_array_equal(sorted_rows(got), sorted_rows(np.array(ref).reshape(-1, 2)))
