@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline runs every n-th query leaf of one pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB once (a plain copy dispatch of "
+                         "known size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -146,6 +149,13 @@ def main():
     elapsed = time.perf_counter() - t_start
     kt = ccc.kernel_times()
     ccc.set_profiling(False)
+
+    if args.copy_calibration_mib:
+        src = torch.empty(args.copy_calibration_mib << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+        torch.cuda.synchronize()
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
 
     name, roof = roofline(kt, step.counts, args.steps, args.iters)
     timed = step.counts[-args.steps * args.iters:]

@@ -281,28 +281,17 @@ template <typename T> __device__ __forceinline__ void add_root(T *slots, double 
 }
 
 // collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes, with the
-// portable cbrt / acos / sin / cos of ccc_libm.h.
-// slots start at -1; only roots inside the add_root window are stored.
-template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef, T *slots) {
-    const double a = coef[0], b = coef[1], c = coef[2], d = coef[3];
-    int n = 0;
-    slots[0] = slots[1] = slots[2] = (T)-1;
-    if (a == 0 && b == 0) {
-        // :532-536 linear case: the root -d/c is computed by the reference but never stored.
-        return;
-    }
-    if (fabs(a) < 1e-16) {
-        // :537-547 quadratic
-        double D = c * c - 4.0 * b * d;
-        if (D >= 0) {
-            D = sqrt(D);
-            double x1 = (-c + D) / (2.0 * b);
-            double x2 = (-c - D) / (2.0 * b);
-            add_root(slots, x1, n);
-            add_root(slots, x2, n);
-        }
-        return;
-    }
+// portable cbrt / acos / sin / cos of ccc_libm.h.  Split in two so that a kernel can bin
+// items by branch between the halves: cubic_prepare classifies and evaluates f, g, h;
+// cubic_finish produces the roots.  solve_cubic = prepare + finish.
+enum : int { CUBIC_NONE = 0, CUBIC_QUADRATIC = 1, CUBIC_TRIPLE = 2, CUBIC_TRIG = 3, CUBIC_CARDANO = 4 };
+
+struct CubicState { double a, b, c, d, f, g, h; };
+
+__device__ __forceinline__ int cubic_prepare(double a, double b, double c, double d, CubicState &st) {
+    st.a = a; st.b = b; st.c = c; st.d = d; st.f = 0; st.g = 0; st.h = 0;
+    if (a == 0 && b == 0) return CUBIC_NONE;   // :532-536 linear root computed, never stored
+    if (fabs(a) < 1e-16) return CUBIC_QUADRATIC;
     const double f = ((3.0 * c / a) - ((b * b) / (a * a))) / 3.0;
     const double b3 = 2 * dd_cube_lead(b);
     const double a2 = a * a;
@@ -313,14 +302,33 @@ template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef,
     const double g = (p1 - p2 + p3) / 27.0;
     const double f3 = dd_cube_lead(f);
     const double h = (g * g) / 4.0 + f3 / 27.0;
-    if (f == 0 && g == 0 && h == 0) {
+    st.f = f; st.g = g; st.h = h;
+    if (f == 0 && g == 0 && h == 0) return CUBIC_TRIPLE;
+    return h <= 0 ? CUBIC_TRIG : CUBIC_CARDANO;
+}
+
+template <typename T> __device__ __forceinline__ void cubic_finish(int code, const CubicState &st, T *slots) {
+    const double a = st.a, b = st.b, c = st.c, d = st.d, g = st.g, h = st.h;
+    int n = 0;
+    slots[0] = slots[1] = slots[2] = (T)-1;
+    if (code == CUBIC_QUADRATIC) {
+        // :537-547
+        double D = c * c - 4.0 * b * d;
+        if (D >= 0) {
+            D = sqrt(D);
+            double x1 = (-c + D) / (2.0 * b);
+            double x2 = (-c - D) / (2.0 * b);
+            add_root(slots, x1, n);
+            add_root(slots, x2, n);
+        }
+    } else if (code == CUBIC_TRIPLE) {
         // :564-575 triple root through a float k
         float k = (float)(d / a);
         double x1;
         if (k >= 0) x1 = (double)(pm_cbrtf(k) * -1);
         else x1 = (double)pm_cbrtf(-k);
         add_root(slots, x1, n);
-    } else if (h <= 0) {
+    } else if (code == CUBIC_TRIG) {
         // :576-591 three real roots (trigonometric form)
         double i = sqrt(((g * g) / 4.0) - h);
         double j = pm_cbrt(i);
@@ -335,7 +343,7 @@ template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef,
         add_root(slots, x1, n);
         add_root(slots, x2, n);
         add_root(slots, x3, n);
-    } else {
+    } else if (code == CUBIC_CARDANO) {
         // :592-611 one real root (Cardano)
         double R = -(g / 2.0) + sqrt(h);
         double S = R >= 0 ? pm_cbrt(R) : -pm_cbrt(-R);
@@ -344,6 +352,12 @@ template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef,
         double x1 = (S + U) - (b / (3.0 * a));
         add_root(slots, x1, n);
     }
+}
+
+template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef, T *slots) {
+    CubicState st;
+    const int code = cubic_prepare(coef[0], coef[1], coef[2], coef[3], st);
+    cubic_finish(code, st, slots);
 }
 
 // Positions at time t: figures.cu:112-117.

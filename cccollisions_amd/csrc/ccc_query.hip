@@ -384,6 +384,20 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
     return p;
 }
 
+// One tile of kThreads items runs in four block phases, each with one record per thread,
+// so that lanes of a wave follow the same branch:
+//   A  item -> tetrahedron -> coefficients -> cubic_prepare (branch, f, g, h); records are
+//      binned by branch in LDS;
+//   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list;
+//   C  filterCollisions per root record (the tetrahedron is gathered again);
+//   D  (impulse mode) accepted collisions wait in an LDS stage; once a block's worth has
+//      gathered, every thread scatters one impulse.
+struct alignas(16) SolveRec {
+    int64_t item;
+    double a, b, c, d, f, g, h;
+};
+constexpr int kRootCap = 3 * kThreads;
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ t0_all, const T *__restrict__ t1_all,
@@ -391,10 +405,16 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   DevCounters *__restrict__ ctr_all) {
+    __shared__ SolveRec s_rec[kThreads];
+    __shared__ unsigned short s_perm[kThreads];
+    __shared__ int s_cnt[5];
+    __shared__ int64_t s_ritem[kRootCap];
+    __shared__ T s_rroot[kRootCap];
+    __shared__ int s_nroot;
     __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
     __shared__ T s_root[MODE == MODE_IMPULSE ? kImpStage : 1];
     __shared__ int s_fill;
-    const int b = blockIdx.y;
+    const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
     const unsigned long long mc = ctr->items;
     const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
@@ -406,81 +426,121 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
     T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
     int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
     unsigned long long nroots = 0, ncoll = 0;
-    if (threadIdx.x == 0) s_fill = 0;
+    if (tid == 0) s_fill = 0;
     __syncthreads();
 
     auto flush = [&](int f) {
-        for (int j = threadIdx.x; j < f; j += kThreads) {
+        for (int j = tid; j < f; j += kThreads) {
             int64_t x, y;
             int type;
             unpack_item(s_item[j], x, y, type);
-            const T *p0 = opaque_ptr(t0), *p1 = opaque_ptr(t1);
-            const MTet<T> mt = gather_mtet(p0, p1, x, y, type);
+            const MTet<T> mt = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
             scatter_impulse(mt, type, s_root[j], x, y, masses, dv, dx, cnt);
         }
     };
 
     for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
-        const int64_t i = tile * kThreads + threadIdx.x;
-        int64_t x = 0, y = 0, it = 0;
-        int type = 0;
-        T acc[3];
-        int nacc = 0;
+        // ---- A: coefficients and branch of one item per thread
+        const int64_t i = tile * kThreads + tid;
+        int code = CUBIC_NONE;
+        CubicState st;
+        int64_t it = 0;
         if (i < M) {
             it = items[i];
+            int64_t x, y;
+            int type;
             unpack_item(it, x, y, type);
-            T cf[4], sl[3];
+            T cf[4];
             {
                 const MTet<T> m = gather_mtet(t0, t1, x, y, type);
                 cubic_coeffs(m, cf);
             }
-            solve_cubic(cf, sl);
-            const bool any = (sl[0] >= 0 && sl[0] <= 1) || (sl[1] >= 0 && sl[1] <= 1) || (sl[2] >= 0 && sl[2] <= 1);
-            if (any) {
-                const T *p0 = opaque_ptr(t0), *p1 = opaque_ptr(t1);
-                const MTet<T> m = gather_mtet(p0, p1, x, y, type);
-#pragma unroll 1
-                for (int s = 0; s < 3; ++s) {
-                    const T r = sl[s];
-                    if (!(r >= 0 && r <= 1)) continue;
+            code = cubic_prepare(cf[0], cf[1], cf[2], cf[3], st);
+        }
+        if (tid < 5) s_cnt[tid] = 0;
+        if (tid == 0) s_nroot = 0;
+        __syncthreads();
+        int pos = 0;
+        if (code != CUBIC_NONE) pos = atomicAdd(&s_cnt[code], 1);
+        __syncthreads();
+        const int o2 = s_cnt[CUBIC_QUADRATIC], o3 = o2 + s_cnt[CUBIC_TRIPLE], o4 = o3 + s_cnt[CUBIC_TRIG];
+        const int tot = o4 + s_cnt[CUBIC_CARDANO];
+        if (code != CUBIC_NONE) {
+            const int p = (code == CUBIC_QUADRATIC ? 0 : code == CUBIC_TRIPLE ? o2 : code == CUBIC_TRIG ? o3 : o4) + pos;
+            s_perm[p] = (unsigned short)tid;
+            SolveRec r;
+            r.item = it; r.a = st.a; r.b = st.b; r.c = st.c; r.d = st.d; r.f = st.f; r.g = st.g; r.h = st.h;
+            s_rec[tid] = r;
+        }
+        __syncthreads();
+        // ---- B: finish the cubic on branch-binned records; roots in [0, 1] to the root list
+        if (tid < tot) {
+            const int c = tid < o2 ? CUBIC_QUADRATIC : tid < o3 ? CUBIC_TRIPLE : tid < o4 ? CUBIC_TRIG : CUBIC_CARDANO;
+            const SolveRec r = s_rec[s_perm[tid]];
+            CubicState rs;
+            rs.a = r.a; rs.b = r.b; rs.c = r.c; rs.d = r.d; rs.f = r.f; rs.g = r.g; rs.h = r.h;
+            T sl[3];
+            cubic_finish(c, rs, sl);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (sl[k] >= 0 && sl[k] <= 1) {
                     ++nroots;
-                    if (filter_collision(m, type, r)) acc[nacc++] = r;
+                    const int p = atomicAdd(&s_nroot, 1);
+                    s_ritem[p] = r.item;
+                    s_rroot[p] = sl[k];
                 }
             }
         }
-        if (MODE == MODE_COLLECT) {
-            unsigned long long slot = block_reserve(&ctr->coll, (unsigned)nacc);
-            int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
-            T *roots = roots_all + (int64_t)b * out_cap;
-            for (int k = 0; k < nacc; ++k, ++slot) {
-                if (slot < (unsigned long long)out_cap) {
+        __syncthreads();
+        // ---- C: filter one root record per thread
+        const int nr = s_nroot;
+        for (int base = 0; base < nr; base += kThreads) {
+            const int j = base + tid;
+            bool ok = false;
+            int64_t rit = 0;
+            T root = 0;
+            if (j < nr) {
+                rit = s_ritem[j];
+                root = s_rroot[j];
+                int64_t x, y;
+                int type;
+                unpack_item(rit, x, y, type);
+                const MTet<T> m = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
+                ok = filter_collision(m, type, root);
+            }
+            if (MODE == MODE_COLLECT) {
+                const unsigned long long slot = block_reserve(&ctr->coll, ok ? 1u : 0u);
+                if (ok && slot < (unsigned long long)out_cap) {
+                    int64_t x, y;
+                    int type;
+                    unpack_item(rit, x, y, type);
+                    int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
                     coll[3 * slot] = x; coll[3 * slot + 1] = y; coll[3 * slot + 2] = type;
-                    roots[slot] = acc[k];
+                    roots_all[(int64_t)b * out_cap + slot] = root;
                 }
+            } else if (ok) {
+                const int p = atomicAdd(&s_fill, 1);
+                s_item[p] = rit;
+                s_root[p] = root;
             }
-        } else {
-            if (nacc) {
-                const int pos = atomicAdd(&s_fill, nacc);
-                for (int k = 0; k < nacc; ++k) {
-                    s_item[pos + k] = it;
-                    s_root[pos + k] = acc[k];
-                }
-            }
+        }
+        // ---- D: scatter staged impulses once a block's worth has gathered
+        if (MODE == MODE_IMPULSE) {
             __syncthreads();
             const int f = s_fill;
             if (f >= kThreads) {
                 flush(f);
-                if (threadIdx.x == 0) ncoll += (unsigned long long)f;
+                if (tid == 0) ncoll += (unsigned long long)f;
                 __syncthreads();
-                if (threadIdx.x == 0) s_fill = 0;
+                if (tid == 0) s_fill = 0;
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
     if (MODE == MODE_IMPULSE) {
         const int f = s_fill;
         flush(f);
-        if (threadIdx.x == 0) ncoll += (unsigned long long)f;
+        if (tid == 0) ncoll += (unsigned long long)f;
     }
     block_add(&ctr->roots, nroots);
     if (MODE == MODE_IMPULSE) block_add(&ctr->coll, ncoll);
