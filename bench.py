@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline runs every n-th query leaf of one pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-sims", type=int, default=0,
+                    help="also time the packed multi-mesh path against the plain per-mesh path on this many "
+                         "folded cloths of 2k-200k triangles (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=1)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy dispatch of "
                          "known size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -77,6 +81,103 @@ class Step:
             c = torch.zeros((self.nv,), dtype=x1.dtype, device=x1.device).index_add_(0, self.flat, k)
             x1 = x1 + torch.where(c[:, None] > 0, s / c.clamp(min=1)[:, None], torch.zeros_like(s))
         return x1
+
+
+class BatchStep:
+    """One CCD + impulse step of many independent meshes of different sizes.  packed=True:
+    each iteration is one compute_impulses_packed call over all meshes (one BVH, one
+    traversal, one narrow phase); packed=False: the plain path, bvh_motion + compute_impulses
+    per mesh.  The vertex update is the same packed index_add in both."""
+
+    def __init__(self, ccc, torch, scenes_list, iters, max_candidates, packed):
+        dev = torch.device("cuda:0")
+        self.ccc, self.torch, self.iters, self.maxc, self.packed = ccc, torch, iters, max_candidates, packed
+        voff = np.cumsum([0] + [sc.x0.shape[0] for sc in scenes_list])
+        self.toff = np.cumsum([0] + [sc.faces.shape[0] for sc in scenes_list]).tolist()
+        faces = np.concatenate([sc.faces + voff[i] for i, sc in enumerate(scenes_list)])
+        self.faces = torch.from_numpy(faces).to(dev)
+        self.flat = self.faces.reshape(-1)
+        self.x0 = torch.from_numpy(np.concatenate([sc.x0 for sc in scenes_list])).to(dev)
+        self.x1 = torch.from_numpy(np.concatenate([sc.x1 for sc in scenes_list])).to(dev)
+        self.t0 = self.x0[self.faces].contiguous()
+        self.masses = torch.from_numpy(np.concatenate([sc.corner_masses() for sc in scenes_list])).to(dev).contiguous()
+        self.nv = self.x0.shape[0]
+        self.counts = []
+        self.first_pass = None      # positions after the very first impulse pass
+
+    def run(self):
+        torch, ccc = self.torch, self.ccc
+        x1 = self.x1
+        for _ in range(self.iters):
+            if self.first_pass is None and self.counts:
+                self.first_pass = x1.clone()
+            t1 = x1[self.faces].contiguous()
+            if self.packed:
+                dv, dx, cnt = ccc.compute_impulses_packed(self.t0, t1, self.masses, self.toff, self.maxc)
+                self.counts.append(ccc.last_counts()[0])
+            else:
+                dxs, cnts, tot = [], [], np.zeros(4, np.int64)
+                for a, b in zip(self.toff[:-1], self.toff[1:]):
+                    t0b, t1b = self.t0[a:b].unsqueeze(0), t1[a:b].unsqueeze(0)
+                    bb, tree = ccc.bvh_motion(t0b, t1b)
+                    _, dxb, cb = ccc.compute_impulses(bb, tree, t0b, t1b, self.masses[a:b].unsqueeze(0), self.maxc, 16)
+                    dxs.append(dxb[0])
+                    cnts.append(cb[0])
+                    tot += np.asarray(ccc.last_counts()[0])
+                dx, cnt = torch.cat(dxs), torch.cat(cnts)
+                self.counts.append(tot.tolist())
+            d = dx.reshape(-1, 3)
+            ok = torch.isfinite(d).all(dim=1)
+            d = torch.where(ok[:, None], d, torch.zeros_like(d))
+            k = torch.where(ok, cnt.reshape(-1).to(x1.dtype), torch.zeros_like(ok, dtype=x1.dtype))
+            s = torch.zeros((self.nv, 3), dtype=x1.dtype, device=x1.device).index_add_(0, self.flat, d)
+            c = torch.zeros((self.nv,), dtype=x1.dtype, device=x1.device).index_add_(0, self.flat, k)
+            x1 = x1 + torch.where(c[:, None] > 0, s / c.clamp(min=1)[:, None], torch.zeros_like(s))
+        return x1
+
+
+def time_steps(torch, step, steps, warmup):
+    for _ in range(warmup):
+        step.run()
+    torch.cuda.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        out = step.run()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
+    return time.perf_counter() - t_start, lat, out
+
+
+def batched_section(ccc, torch, scenes, args):
+    """Folded cloths of 2k-200k triangles (log-uniform sizes, seeded): the packed path
+    against the plain per-mesh path on the same simulation.  Reports how far apart the
+    positions of the two paths are after their first pass next to the largest correction of
+    that pass."""
+    sims = scenes.batched_cloths(args.batched_sims)
+    tris = int(sum(sc.num_triangles for sc in sims))
+    packed = BatchStep(ccc, torch, sims, args.iters, args.max_candidates, True)
+    plain = BatchStep(ccc, torch, sims, args.iters, args.max_candidates, False)
+    tp, latp, _ = time_steps(torch, packed, args.batched_steps, 1)
+    tq, latq, _ = time_steps(torch, plain, args.batched_steps, 1)
+    # after one pass from the same input the two paths differ only by the order of the fp64
+    # atomic sums; later passes amplify such differences through borderline collisions
+    moved = float((plain.first_pass - plain.x1).abs().max())
+    diff = float((packed.first_pass - plain.first_pass).abs().max())
+    return {
+        "sims": len(sims), "triangles": tris,
+        "sizes": [int(sc.num_triangles) for sc in sims],
+        "steps": args.batched_steps, "impulse_iterations": args.iters,
+        "packed_steps_per_s": args.batched_steps / tp, "plain_steps_per_s": args.batched_steps / tq,
+        "packed_over_plain": tq / tp,
+        "packed_p99_ms": 1e3 * float(np.percentile(latp, 99)), "plain_p99_ms": 1e3 * float(np.percentile(latq, 99)),
+        "mean_candidates_per_iter": float(np.mean([c[0] for c in packed.counts])),
+        "mean_collisions_per_iter": float(np.mean([c[3] for c in packed.counts])),
+        # the first pass of both runs sees the same input: equal K, M, V, R over the batch
+        "first_pass_counts_equal": list(map(int, packed.counts[0])) == list(map(int, plain.counts[0])),
+        "first_pass_max_position_difference": diff, "first_pass_max_correction": moved,
+    }
 
 
 def roofline(kt, counts, steps, iters):
@@ -189,6 +290,8 @@ def main():
         "roofline": roof,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items())},
     }
+    if args.batched_sims:
+        out["batched"] = batched_section(ccc, torch, scenes, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_stride, args.iters)
     print(json.dumps(out), flush=True)

@@ -33,6 +33,7 @@ find_mincol_cand = _ext.find_mincol_cand
 compute_impulses = _ext.compute_impulses
 compute_impulses_partial = _ext.compute_impulses_partial
 compute_impulses_from_collisions = _ext.compute_impulses_from_collisions
+compute_impulses_packed = _ext.compute_impulses_packed
 set_fixed_capacity = _ext.set_fixed_capacity
 last_counts = _ext.last_counts
 set_profiling = _ext.set_profiling
@@ -46,4 +47,7 @@ ENTRY_POINTS = (
     "compute_impulses_from_collisions",
 )
 
-__all__ = list(ENTRY_POINTS) + ["set_fixed_capacity", "last_counts", "set_profiling", "kernel_times"]
+# beyond the reference's API: many meshes of different sizes in one pass of launches
+EXTENSIONS = ("compute_impulses_packed",)
+
+__all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "last_counts", "set_profiling", "kernel_times"]

@@ -329,6 +329,13 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     return CCC_OK;
 }
 
+ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
+                            void *bboxes, int64_t *tree, hipStream_t s) {
+    if (dtype == CCC_FLOAT64) return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, (double *)bboxes, tree, s);
+    if (dtype == CCC_FLOAT32) return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, (float *)bboxes, tree, s);
+    return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+}
+
 static ccc_status build_entry(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
                               void *bboxes, int64_t *tree, void *stream) {
     clear_error();
@@ -338,10 +345,7 @@ static ccc_status build_entry(ccc_context *ctx, const void *t0, const void *t1, 
     if (B == 0) return CCC_OK;
     if (!t0 || !bboxes || !tree) return bad_arg("NULL tensor pointer");
     CCC_HIP_TRY(hipSetDevice(ctx->device));
-    hipStream_t s = (hipStream_t)stream;
-    if (dtype == CCC_FLOAT64) return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, (double *)bboxes, tree, s);
-    if (dtype == CCC_FLOAT32) return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, (float *)bboxes, tree, s);
-    return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+    return build_bvh_device(ctx, t0, t1, B, n, dtype, bboxes, tree, (hipStream_t)stream);
 }
 
 }  // namespace ccc

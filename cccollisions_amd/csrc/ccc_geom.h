@@ -400,9 +400,15 @@ template <typename T> __device__ __forceinline__ bool filter_collision(const MTe
 // Impulse of one accepted collision: collisions_impulses.cu:24-123 (mass quad),
 // :162-543 (coefficients, normal, normal velocity, mass pair), :936-1050 (magnitudes) and
 // :609-629 (dx extrapolation).  imp[k] / imp_dx[k] go to tetrahedron corner k.
+// What one collision contributes: the unit normal n, the two impulse magnitudes and the
+// position factor 1 - max(root - offset, 0).
+template <typename T> struct ImpulseParams {
+    V3<T> nrm;
+    T im0, im1, keep;
+};
+
 template <typename T>
-__device__ __forceinline__ void collision_impulse(const MTet<T> &m, int type, T root, const T *q,
-                                                  V3<T> *imp, V3<T> *imp_dx) {
+__device__ __forceinline__ ImpulseParams<T> collision_impulse(const MTet<T> &m, int type, T root, const T *q) {
     V3<T> p[4];
     tet_at(m, root, p);
     V3<T> nrm;
@@ -454,16 +460,24 @@ __device__ __forceinline__ void collision_impulse(const MTet<T> &m, int type, T 
     T mf0, mf1;
     if (tabs(msum) < 1e-15) { mf0 = (T)0.5; mf1 = (T)0.5; }
     else { mf0 = mpx / msum; mf1 = 1 - mf0; }
-    const T im0 = nvel * mf1;
-    const T im1 = -nvel * mf0;
+    ImpulseParams<T> ip;
+    ip.nrm = nrm;
+    ip.im0 = nvel * mf1;
+    ip.im1 = -nvel * mf0;
     const T root_offset = (T)fmax((double)(root - offset), 0.);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        bool second = type < 9 ? (k >= 2) : (type < 12 ? (k == 3) : (k >= 1));
-        imp[k] = nrm * (second ? im1 : im0);
-        V3<T> nvk = m.dv[k] + imp[k];
-        imp_dx[k] = (nvk - m.dv[k]) * (1 - root_offset);
-    }
+    ip.keep = 1 - root_offset;
+    return ip;
+}
+
+// Impulse of tetrahedron corner k: n * imp0 for the first primitive's corners, n * imp1 for
+// the second's (collisions_impulses.cu:1017-1032); its position correction
+// dx = ((dv + imp) - dv) * (1 - max(root - offset, 0)) (:618-629).
+template <typename T>
+__device__ __forceinline__ void corner_impulse(const ImpulseParams<T> &ip, int type, int k, V3<T> dvk, V3<T> &imp, V3<T> &imd) {
+    const bool second = type < 9 ? (k >= 2) : (type < 12 ? (k == 3) : (k >= 1));
+    imp = ip.nrm * (second ? ip.im1 : ip.im0);
+    const V3<T> nvk = dvk + imp;
+    imd = (nvk - dvk) * ip.keep;
 }
 
 // proximity_static.cu:205-228 distance_point_to_segment
@@ -544,6 +558,46 @@ template <typename T> __device__ __forceinline__ bool sat_overlap(const V3<T> *a
         flag = flag && ((l1 <= h2 + thr) && (h1 >= l2 - thr));
     }
     return flag;
+}
+
+// figures.cu:605-610 hasVertex: exact coordinate match with one of the corners.
+template <typename T> __device__ __forceinline__ bool has_vertex(const V3<T> *t, V3<T> v) {
+    return veq(t[0], v) || veq(t[1], v) || veq(t[2], v);
+}
+
+// collisions_static.cu:345-442 addTECollsion: does edge (e0, e1) of one triangle pierce
+// triangle t?  Returns true with (loop, coef) when the reference emits a row.  An edge
+// that starts on a corner of t (cid < 3 only) is a loop node with coefficient 0; NaN
+// passes both the parameter and the barycentric tests, as in the reference.
+template <typename T>
+__device__ __forceinline__ bool te_intersect(const V3<T> *t, V3<T> e0, V3<T> e1, int cid, int64_t &loop, T &coef) {
+    const V3<T> edge1 = t[1] - t[0], edge2 = t[2] - t[0];
+    const V3<T> nrm = normalize(cross(edge1, edge2));
+    const bool s0 = has_vertex(t, e0), s1 = has_vertex(t, e1);
+    if (s0 || s1) {
+        if (cid > 2 || !s0) return false;
+        loop = 0;
+        coef = (T)0;
+        return true;
+    }
+    const V3<T> dir = e1 - e0;
+    const T en = dot(dir, nrm);
+    if (en == (T)0) return false;
+    const T d = dot(nrm, t[0]);
+    const T tp = (d - dot(nrm, e0)) / en;
+    if (tp < (T)0 || tp > (T)1) return false;
+    const V3<T> ep = (e0 + dir * tp) - t[0];
+    const T d00 = dot(edge1, edge1), d01 = dot(edge1, edge2), d11 = dot(edge2, edge2);
+    const T d20 = dot(ep, edge1), d21 = dot(ep, edge2);
+    const T denom = d00 * d11 - d01 * d01;
+    if (denom < (T)1e-15) return false;
+    const T v = (d11 * d20 - d01 * d21) / denom;
+    const T w = (d00 * d21 - d01 * d20) / denom;
+    const T u = (T)1.0 - v - w;
+    if (u < (T)0 || v < (T)0 || w < (T)0) return false;
+    loop = -1;
+    coef = tp;
+    return true;
 }
 
 }  // namespace ccc

@@ -133,13 +133,20 @@ struct alignas(64) DevCounters {
     unsigned long long items;   // M
     unsigned long long roots;   // V
     unsigned long long coll;    // R (total, may exceed the output capacity)
+    unsigned long long items_done;  // M of the pair chunks already finished
     unsigned int flags;         // DEV_* bits
-    unsigned int pad[7];
+    unsigned int pad[5];
 };
+static_assert(sizeof(DevCounters) == 64, "DevCounters is one 64-byte line");
 
 // Fill kernels (used instead of hipMemsetAsync so that the calls stay capturable).
 ccc_status fill_u64(void *p, int64_t count, uint64_t value, hipStream_t s);
 ccc_status fill_u32(void *p, int64_t count, uint32_t value, hipStream_t s);
+
+// BVH over B soups of n triangles each (t1 = NULL: static boxes) into the wire format;
+// arguments already validated, the device already current.
+ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
+                            void *bboxes, int64_t *tree, hipStream_t s);
 
 }  // namespace ccc
 

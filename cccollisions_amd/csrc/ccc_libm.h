@@ -7,61 +7,96 @@
 // from correctly rounded IEEE operations (+ - * / sqrt fma frexp ldexp); results agree bit
 // for bit between the two.  Accuracy: cbrt correctly rounded on all sampled inputs, acos,
 // sin and cos within 1 ulp (pinned against mpmath on the CPU copy).
-//   cbrt : range reduction to m in [0.5, 4), linear start, three Halley steps
-//          y <- y (y^3 + 2m) / (2y^3 + m), one Newton correction with an fma residual.
+//   cbrt : x = m0 2^e, m0 in [1/2, 1), e = 3q + r; w ~ m^(-1/3) for m = m0 2^r from a
+//          degree-4 polynomial in m0 times 2^(-r/3), two division-free Newton steps
+//          w <- w + w (1 - m w^3) / 3, then y = m w^2 and one Newton correction
+//          y <- y - (y^3 - m) w^2 / 3 with the residual y^3 - m formed exactly by fma.
 //   acos : |x| <= 1/2: pi/2 - asin(x); |x| > 1/2: 2 asin(sqrt((1-|x|)/2)) (mirrored with
-//          pi for x < 0); asin(z) = z + z^3 sum c_n z^(2n-2), c_n = (2n)!/(4^n (n!)^2 (2n+1)),
-//          27 terms; pi and pi/2 as double-double.
-//   sin, cos : Taylor series, 11 terms, for |y| <= 1.1 (the solver's arguments lie in
-//          [0, pi/3]).
-// Coefficients are the exact rationals rounded to double (hex literals).  They are read
-// through volatile pointers (scalar loads from the constant tables): otherwise the
-// compiler hoists all of them out of the persistent item loops into ~100 VGPRs.
+//          pi for x < 0); asin(z) = z + z^3 P(z^2), P of degree 13 on [0, 1/4]; pi and
+//          pi/2 as double-double.
+//   sin, cos : sin(y) = y + y^3 S(y^2), cos(y) = 1 + y^2 C(y^2), S and C of degree 6 and 7
+//          on y^2 in [0, 1.1] (the solver's arguments lie in [0, pi/3]).
+// The polynomials are Chebyshev interpolants rounded to double (gen_libm_coeffs.py prints
+// them), materialised in SGPRs at their point of use.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace ccc {
 
-__device__ const double CCC_ASIN_C[27] = {
-    0x1.5555555555555p-3, 0x1.3333333333333p-4, 0x1.6db6db6db6db7p-5, 0x1.f1c71c71c71c7p-6,
-    0x1.6e8ba2e8ba2e9p-6, 0x1.1c4ec4ec4ec4fp-6, 0x1.c99999999999ap-7, 0x1.7a87878787878p-7,
-    0x1.3fde50d79435ep-7, 0x1.12ef3cf3cf3cfp-7, 0x1.df3bd37a6f4dfp-8, 0x1.a6863d70a3d71p-8,
-    0x1.782dda12f684cp-8, 0x1.51ba308d3dcb1p-8, 0x1.31683bdef7bdfp-8, 0x1.15ee9d45d1746p-8,
-    0x1.fcaf8fb6db6dbp-9, 0x1.d3d2a8e0dd67dp-9, 0x1.b026f57b13b14p-9, 0x1.90cb77f60c7cep-9,
-    0x1.750de64d7d05fp-9, 0x1.5c5f56efaaaabp-9, 0x1.464c0950f7d47p-9, 0x1.3275586c5f2f0p-9,
-    0x1.208d3570ae5a6p-9, 0x1.1052bc5fa960ap-9, 0x1.018f963c229bfp-9};
-__device__ const double CCC_SIN_C[11] = {
-    -0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13, 0x1.71de3a556c734p-19,
-    -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33, -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49,
-    -0x1.2f49b46814157p-57, 0x1.71b8ef6dcf572p-66, -0x1.761b41316381ap-75};
-__device__ const double CCC_COS_C[11] = {
-    -0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16,
-    -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29, -0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-45,
-    -0x1.6827863b97d97p-53, 0x1.e542ba4020225p-62, -0x1.0ce396db7f853p-70};
+// Polynomial coefficients, constant term first (constant-evaluated only; see sgpr_const).
+struct AsinCoef {
+    static constexpr double c[14] = {
+        0x1.5555555555555p-3, 0x1.3333333333388p-4, 0x1.6db6db6dac1e0p-5, 0x1.f1c71c7a52ba3p-6,
+        0x1.6e8ba123e494cp-6, 0x1.1c4efce23019fp-6, 0x1.c990ad3d8fdcap-7, 0x1.7b027ee1dd585p-7,
+        0x1.3b49de7121487p-7, 0x1.31622469ce5adp-7, 0x1.8f193743418ffp-9, 0x1.406192d124629p-6,
+        -0x1.3b416bb7d9257p-6, 0x1.e529c6fce9bb4p-6};
+};
+struct SinCoef {
+    static constexpr double c[7] = {
+        -0x1.5555555555555p-3, 0x1.11111111110efp-7, -0x1.a01a01a012493p-13, 0x1.71de3a4ae46ddp-19,
+        -0x1.ae6448cfef0b7p-26, 0x1.611b3f22095d8p-33, -0x1.a871f0b737686p-41};
+};
+struct CosCoef {
+    static constexpr double c[8] = {
+        -0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16bafp-10, 0x1.a01a01a00e3e7p-16,
+        -0x1.27e4fb6d0860ep-22, 0x1.1eed84cdda596p-29, -0x1.938c53aa9fa3ap-37, 0x1.a85969626b78bp-45};
+};
+struct RcbrtCoef {
+    static constexpr double c[5] = {
+        0x1.0cb878fb9eb85p+1, -0x1.76eb5dac744b0p+1, 0x1.b9a7cd80a5ca6p+1, -0x1.1724277eb1344p+1,
+        0x1.1ec0fe5370dbbp-1};
+};
+#define CCC_CBRT_SCALE1 0x1.965fea53d6e3dp-1   // 2^(-1/3)
+#define CCC_CBRT_SCALE2 0x1.428a2f98d728bp-1   // 2^(-2/3)
+#define CCC_THIRD 0x1.5555555555555p-2
 #define CCC_PI_HI 0x1.921fb54442d18p+1
 #define CCC_PI_LO 0x1.1a62633145c07p-53
 #define CCC_PIO2_HI 0x1.921fb54442d18p+0
 #define CCC_PIO2_LO 0x1.1a62633145c07p-54
 
+// A double constant materialised in an SGPR pair by two s_mov_b32 immediates.  The asm
+// statements are volatile, so the compiler keeps them where they are used instead of
+// hoisting every coefficient of the persistent kernels' loops into VGPRs.
+template <uint64_t B> __device__ __forceinline__ double sgpr_const() {
+    uint32_t lo, hi;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((uint32_t)(B & 0xffffffffull)));
+    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((uint32_t)(B >> 32)));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+template <class TAB, int K> __device__ __forceinline__ double coef() {
+    return sgpr_const<__builtin_bit_cast(uint64_t, TAB::c[K])>();
+}
+// p <- fma(p, t, c[k]) for k = K, K-1, ..., 0
+template <class TAB, int K> __device__ __forceinline__ double horner(double p, double t) {
+    if constexpr (K < 0) return p;
+    else return horner<TAB, K - 1>(fma(p, t, coef<TAB, K>()), t);
+}
+
 __device__ __forceinline__ double pm_cbrt(double x) {
     if (x == 0.0 || !isfinite(x)) return x;
     const double ax = fabs(x);
     int e;
-    double m = frexp(ax, &e);
+    const double m0 = frexp(ax, &e);
     int q = e / 3, r = e - 3 * q;
     if (r < 0) { r += 3; q -= 1; }
-    m = ldexp(m, r);
-    double y = 0.6807 + 0.2266 * m;
+    double w = horner<RcbrtCoef, 3>(coef<RcbrtCoef, 4>(), m0);
+    w = w * (r == 0 ? 1.0 : r == 1 ? sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE1)>()
+                                   : sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE2)>());
+    const double m = ldexp(m0, r);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double y3 = y * y * y;
-        y = y * (y3 + 2.0 * m) / (2.0 * y3 + m);
+    for (int k = 0; k < 2; ++k) {
+        const double w3 = (w * w) * w;
+        const double d = fma(-m, w3, 1.0);
+        w = fma(w * d, CCC_THIRD, w);
     }
+    const double w2 = w * w;
+    double y = m * w2;
     const double y2 = y * y, y2e = fma(y, y, -y2);
     const double y3 = y2 * y, y3e = fma(y2, y, -y3);
     const double res = (y3 - m) + (y3e + y2e * y);
-    y = y - res / (3.0 * y2);
+    y = fma(-res, w2 * CCC_THIRD, y);
     y = ldexp(y, q);
     return x < 0 ? -y : y;
 }
@@ -70,10 +105,7 @@ __device__ __forceinline__ float pm_cbrtf(float x) { return (float)pm_cbrt((doub
 
 __device__ __forceinline__ double pm_asin_small(double z) {
     const double t = z * z;
-    const volatile double *C = CCC_ASIN_C;
-    double p = C[26];
-#pragma unroll
-    for (int k = 25; k >= 0; --k) p = fma(p, t, C[k]);
+    const double p = horner<AsinCoef, 12>(coef<AsinCoef, 13>(), t);
     return fma(z * t, p, z);
 }
 
@@ -97,19 +129,13 @@ __device__ __forceinline__ double pm_acos(double x) {
 
 __device__ __forceinline__ double pm_sin(double y) {
     const double t = y * y;
-    const volatile double *C = CCC_SIN_C;
-    double p = C[10];
-#pragma unroll
-    for (int k = 9; k >= 0; --k) p = fma(p, t, C[k]);
+    const double p = horner<SinCoef, 5>(coef<SinCoef, 6>(), t);
     return fma(y * t, p, y);
 }
 
 __device__ __forceinline__ double pm_cos(double y) {
     const double t = y * y;
-    const volatile double *C = CCC_COS_C;
-    double p = C[10];
-#pragma unroll
-    for (int k = 9; k >= 0; --k) p = fma(p, t, C[k]);
+    const double p = horner<CosCoef, 6>(coef<CosCoef, 7>(), t);
     return fma(t, p, 1.0);
 }
 

@@ -10,6 +10,7 @@
 //   -> per pair: 15-bit sub-primitive mask -> packed (x, y, type) items -> per item:
 //   narrow phase.
 #include <float.h>
+#include <limits.h>
 
 #include <algorithm>
 
@@ -187,6 +188,7 @@ template <typename T, bool PARTIAL>
 __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
                                                            const TNode *__restrict__ nodes_all, int64_t n, T thr,
                                                            const uint8_t *__restrict__ to_check_all,
+                                                           const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
                                                            DevCounters *__restrict__ ctr_all) {
     __shared__ int stack[kStackDepth][kTravThreads];
@@ -213,6 +215,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
         const int qid = (int)tree[4 * qrow + 3];
         const float qx0 = f32_down((double)q.mn.x), qy0 = f32_down((double)q.mn.y), qz0 = f32_down((double)q.mn.z);
         const float qx1 = f32_up((double)q.mx.x), qy1 = f32_up((double)q.mx.y), qz1 = f32_up((double)q.mx.z);
+        // packed meshes: leaves at positions >= lim belong to later meshes
+        const int lim = mesh_end ? mesh_end[p] : INT_MAX;
         int sp = 0, node = 0;
         while (true) {
             const TNode nd = nodes[node];
@@ -222,6 +226,10 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                 ol = ol && nd.d.z > (int)p;
                 orr = orr && nd.d.w > (int)p;
             }
+            // the right child's first leaf follows the left child's last one (trees built
+            // here cover contiguous position ranges); a node is only entered when its first
+            // leaf is below lim, so the left child never needs the test
+            orr = orr && nd.d.z + 1 < lim;
             int oid0 = -1, oid1 = -1;
             if (ol && nd.d.x < 0) {
                 const int64_t lrow = n - 1 + (int64_t)(~nd.d.x);
@@ -292,20 +300,21 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 // MODE 1: static with threshold (proximity_static.cu:11-118) -> items.
 // MODE 2: moving, written as (x, y, type) rows (find_minimal_candidates, :321-338).
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK,
+__global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
                                                       const T *__restrict__ t0_all, const T *__restrict__ t1_all,
                                                       int64_t n, T thr, int64_t *__restrict__ items_all, int64_t capM,
                                                       DevCounters *__restrict__ ctr_all) {
     const int b = blockIdx.y;
     DevCounters *ctr = ctr_all + b;
     const unsigned long long kc = ctr->cand;
-    const int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
     const int2 *pairs = pairs_all + (int64_t)b * capK;
     const T *t0 = t0_all + (int64_t)b * n * 9;
     const T *t1 = MODE == 1 ? nullptr : t1_all + (int64_t)b * n * 9;
     int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
-    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
-        const int64_t i = tile * kThreads + threadIdx.x;
+    for (int64_t tile = blockIdx.x; c0 + tile * kThreads < K; tile += gridDim.x) {
+        const int64_t i = c0 + tile * kThreads + threadIdx.x;
         uint32_t mask = 0;
         int x = 0, y = 0;
         if (i < K) {
@@ -362,18 +371,109 @@ __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T ro
         type_corner(type, k, side[k], idx[k]);
         q[k] = masses[3 * (side[k] ? y : x) + idx[k]];
     }
-    V3<T> imp[4], imd[4];
-    collision_impulse(m, type, root, q, imp, imd);
+    const ImpulseParams<T> ip = collision_impulse(m, type, root, q);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
+        V3<T> imp, imd;
+        corner_impulse(ip, type, k, m.dv[k], imp, imd);
         const int64_t c = 3 * (side[k] ? y : x) + idx[k];
-        atomicAdd(dv + 3 * c + 0, imp[k].x);
-        atomicAdd(dv + 3 * c + 1, imp[k].y);
-        atomicAdd(dv + 3 * c + 2, imp[k].z);
-        atomicAdd(dx + 3 * c + 0, imd[k].x);
-        atomicAdd(dx + 3 * c + 1, imd[k].y);
-        atomicAdd(dx + 3 * c + 2, imd[k].z);
+        atomicAdd(dv + 3 * c + 0, imp.x);
+        atomicAdd(dv + 3 * c + 1, imp.y);
+        atomicAdd(dv + 3 * c + 2, imp.z);
+        atomicAdd(dx + 3 * c + 0, imd.x);
+        atomicAdd(dx + 3 * c + 1, imd.y);
+        atomicAdd(dx + 3 * c + 2, imd.z);
         atomicAdd(cnt + c, 1);
+    }
+}
+
+// Per-block sums of corner impulses: open addressing on the corner key 3 * triangle +
+// corner, kImpSlots slots (a power of two), linear probing up to kImpProbe slots; a
+// contribution that finds no slot goes to the global arrays directly.
+constexpr int kImpSlots = 512;
+constexpr int kImpProbe = 16;
+
+template <typename T> struct ImpTable {
+    int *key;
+    int *count;
+    T *acc;     // [slot][dv.xyz, dx.xyz]
+
+    static constexpr size_t bytes() { return (size_t)kImpSlots * (2 * sizeof(int) + 6 * sizeof(T)); }
+    __device__ static ImpTable over(unsigned char *raw) {
+        ImpTable t;
+        t.acc = reinterpret_cast<T *>(raw);
+        t.key = reinterpret_cast<int *>(raw + (size_t)kImpSlots * 6 * sizeof(T));
+        t.count = t.key + kImpSlots;
+        return t;
+    }
+    __device__ void clear(int tid) {
+        for (int s = tid; s < kImpSlots; s += kThreads) {
+            key[s] = -1;
+            count[s] = 0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[6 * s + c] = (T)0;
+        }
+    }
+    // slot of `k`, inserting it if absent; -1 when the probe window is full
+    __device__ int slot(int k) {
+        int h = (int)(((unsigned)k * 2654435761u) >> (32 - 9)) & (kImpSlots - 1);
+        for (int p = 0; p < kImpProbe; ++p) {
+            const int old = atomicCAS(&key[h], -1, k);
+            if (old == -1 || old == k) return h;
+            h = (h + 1) & (kImpSlots - 1);
+        }
+        return -1;
+    }
+    __device__ void drain(int tid, T *dv, T *dx, int *cnt) {
+        for (int s = tid; s < kImpSlots; s += kThreads) {
+            const int k = key[s];
+            if (k < 0) continue;
+            atomicAdd(dv + 3 * (int64_t)k + 0, acc[6 * s + 0]);
+            atomicAdd(dv + 3 * (int64_t)k + 1, acc[6 * s + 1]);
+            atomicAdd(dv + 3 * (int64_t)k + 2, acc[6 * s + 2]);
+            atomicAdd(dx + 3 * (int64_t)k + 0, acc[6 * s + 3]);
+            atomicAdd(dx + 3 * (int64_t)k + 1, acc[6 * s + 4]);
+            atomicAdd(dx + 3 * (int64_t)k + 2, acc[6 * s + 5]);
+            atomicAdd(cnt + k, count[s]);
+        }
+    }
+};
+static_assert(kImpSlots == 1 << 9, "the slot hash takes 9 bits");
+
+template <typename T>
+__device__ __forceinline__ void scatter_impulse_lds(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
+                                                    ImpTable<T> &tab, T *dv, T *dx, int *cnt) {
+    T q[4];
+    int side[4], idx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        type_corner(type, k, side[k], idx[k]);
+        q[k] = masses[3 * (side[k] ? y : x) + idx[k]];
+    }
+    const ImpulseParams<T> ip = collision_impulse(m, type, root, q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        V3<T> imp, imd;
+        corner_impulse(ip, type, k, m.dv[k], imp, imd);
+        const int64_t c = 3 * (side[k] ? y : x) + idx[k];
+        const int s = tab.slot((int)c);
+        if (s >= 0) {
+            atomicAdd(&tab.acc[6 * s + 0], imp.x);
+            atomicAdd(&tab.acc[6 * s + 1], imp.y);
+            atomicAdd(&tab.acc[6 * s + 2], imp.z);
+            atomicAdd(&tab.acc[6 * s + 3], imd.x);
+            atomicAdd(&tab.acc[6 * s + 4], imd.y);
+            atomicAdd(&tab.acc[6 * s + 5], imd.z);
+            atomicAdd(&tab.count[s], 1);
+        } else {
+            atomicAdd(dv + 3 * c + 0, imp.x);
+            atomicAdd(dv + 3 * c + 1, imp.y);
+            atomicAdd(dv + 3 * c + 2, imp.z);
+            atomicAdd(dx + 3 * c + 0, imd.x);
+            atomicAdd(dx + 3 * c + 1, imd.y);
+            atomicAdd(dx + 3 * c + 2, imd.z);
+            atomicAdd(cnt + c, 1);
+        }
     }
 }
 
@@ -405,11 +505,14 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   DevCounters *__restrict__ ctr_all) {
-    __shared__ SolveRec s_rec[kThreads];
+    // s_rec (phases A-B) and the root list (phases B-C) share their LDS with the impulse
+    // table of phase D
+    __shared__ __align__(16) unsigned char s_raw[sizeof(SolveRec) * kThreads + (sizeof(int64_t) + sizeof(T)) * kRootCap];
+    SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
+    int64_t *s_ritem = reinterpret_cast<int64_t *>(s_raw + sizeof(SolveRec) * kThreads);
+    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kThreads + sizeof(int64_t) * kRootCap);
     __shared__ unsigned short s_perm[kThreads];
     __shared__ int s_cnt[5];
-    __shared__ int64_t s_ritem[kRootCap];
-    __shared__ T s_rroot[kRootCap];
     __shared__ int s_nroot;
     __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
     __shared__ T s_root[MODE == MODE_IMPULSE ? kImpStage : 1];
@@ -429,13 +532,26 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
     if (tid == 0) s_fill = 0;
     __syncthreads();
 
+    // Staged collisions are scattered kThreads at a time: their corner impulses are first
+    // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
+    // dx / cnt with one set of global atomics.
+    static_assert(ImpTable<T>::bytes() <= sizeof(s_raw), "the impulse table fits the phase A-C buffers");
+    ImpTable<T> tab = ImpTable<T>::over(s_raw);
     auto flush = [&](int f) {
-        for (int j = tid; j < f; j += kThreads) {
-            int64_t x, y;
-            int type;
-            unpack_item(s_item[j], x, y, type);
-            const MTet<T> mt = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
-            scatter_impulse(mt, type, s_root[j], x, y, masses, dv, dx, cnt);
+        for (int base = 0; base < f; base += kThreads) {
+            tab.clear(tid);
+            __syncthreads();
+            const int j = base + tid;
+            if (j < f) {
+                int64_t x, y;
+                int type;
+                unpack_item(s_item[j], x, y, type);
+                const MTet<T> mt = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
+                scatter_impulse_lds(mt, type, s_root[j], x, y, masses, tab, dv, dx, cnt);
+            }
+            __syncthreads();
+            tab.drain(tid, dv, dx, cnt);
+            __syncthreads();
         }
     };
 
@@ -641,7 +757,108 @@ __global__ __launch_bounds__(kThreads) void k_sat(const int2 *__restrict__ pairs
 
 // ---- host orchestration ---------------------------------------------------------------------
 
-enum class Narrow { SAT, PROXIMITY, CCD_COLLECT, CCD_IMPULSE, MINCOL };
+// Triangle-edge queries over the candidate pairs of a threshold-0 traversal, one pair per
+// thread: SAT with at most `allow` shared vertices (collisions_static.cu:123-159), then the
+// six edges cid 0..2 = (v0v1, v1v2, v2v0) of x against y and 3..5 of y against x.
+//   TE_COLLISIONS  (allow 0) rows (x, y, cid, loop) + edge coefficient for every edge that
+//                  pierces the other triangle (collisions_static.cu:445-500, :612-779);
+//   TE_CANDIDATES  (allow 1) rows (x, y, cid, loop) for every edge that does not end on a
+//                  shared corner (collisions_static.cu:520-604, :782-943);
+//   TE_CANDIDATES2 (allow 1) one row (x, y) per pair with bit masks (mx, my) of the same
+//                  edges (icontour.cu:11-108, :110-282).
+// A thread reserves all of its rows with one block-wide reservation.
+enum { TE_COLLISIONS = 0, TE_CANDIDATES = 1, TE_CANDIDATES2 = 2 };
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_tri_edge(const int2 *__restrict__ pairs_all, int64_t capK,
+                                                       const T *__restrict__ t0_all, int64_t n,
+                                                       int64_t *__restrict__ rows_all, T *__restrict__ coef_all,
+                                                       int32_t *__restrict__ masks_all, int64_t out_cap,
+                                                       DevCounters *__restrict__ ctr_all) {
+    const int b = blockIdx.y;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long kc = ctr->cand;
+    const int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    const int2 *pairs = pairs_all + (int64_t)b * capK;
+    const T *t0 = t0_all + (int64_t)b * n * 9;
+    constexpr int W = MODE == TE_CANDIDATES2 ? 2 : 4;
+    int64_t *rows = rows_all + (int64_t)b * out_cap * W;
+    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
+        const int64_t i = tile * kThreads + threadIdx.x;
+        int x = 0, y = 0;
+        unsigned emit = 0;          // bit cid: row for edge cid
+        unsigned loops = 0;         // bit cid: loop node (loop id 0)
+        int mx = 0, my = 0;
+        T coefs[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) coefs[c] = (T)-1;
+        bool valid = false;
+        if (i < K) {
+            const int2 pr = pairs[i];
+            x = pr.x;
+            y = pr.y;
+            V3<T> a[3], c3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c3[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
+            const int allow = MODE == TE_COLLISIONS ? 0 : 1;
+            valid = sat_overlap(a, c3, (T)0) && !(count_shared(a, c3) > allow);
+            if (valid) {
+#pragma unroll
+                for (int cid = 0; cid < 6; ++cid) {
+                    const V3<T> *t1 = cid < 3 ? a : c3;
+                    const V3<T> *t2 = cid < 3 ? c3 : a;
+                    const int e = cid % 3;
+                    const V3<T> e0 = t1[e], e1 = t1[(e + 1) % 3];
+                    if (MODE == TE_COLLISIONS) {
+                        int64_t loop;
+                        T cf;
+                        if (te_intersect(t2, e0, e1, cid, loop, cf)) {
+                            emit |= 1u << cid;
+                            if (loop == 0) loops |= 1u << cid;
+                            coefs[cid] = cf;
+                        }
+                    } else {
+                        const bool s0 = has_vertex(t2, e0), s1 = has_vertex(t2, e1);
+                        if (s0) {
+                            if (cid < 3) { emit |= 1u << cid; loops |= 1u << cid; mx |= 1 << cid; my |= 1 << cid; }
+                        } else if (!s1) {
+                            // the reference compares t2 with the box of its own edge (collisions_static.cu:594-596)
+                            const Box<T> b0 = tri_box(t2[0], t2[1], t2[2]);
+                            const Box<T> b1 = unite(point_box(t2[e]), point_box(t2[(e + 1) % 3]));
+                            if (overlap(b0, b1, (T)0)) { emit |= 1u << cid; mx |= 1 << cid; }
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned cnt = MODE == TE_CANDIDATES2 ? (valid ? 1u : 0u) : (unsigned)__popc(emit);
+        unsigned long long slot = block_reserve(&ctr->coll, cnt);
+        if (MODE == TE_CANDIDATES2) {
+            if (valid && slot < (unsigned long long)out_cap) {
+                rows[2 * slot] = x;
+                rows[2 * slot + 1] = y;
+                int32_t *masks = masks_all + (int64_t)b * out_cap * 2;
+                masks[2 * slot] = mx;
+                masks[2 * slot + 1] = my;
+            }
+        } else {
+#pragma unroll
+            for (int cid = 0; cid < 6; ++cid) {
+                if (!(emit & (1u << cid))) continue;
+                if (slot < (unsigned long long)out_cap) {
+                    rows[4 * slot] = x;
+                    rows[4 * slot + 1] = y;
+                    rows[4 * slot + 2] = cid;
+                    rows[4 * slot + 3] = (loops & (1u << cid)) ? 0 : -1;
+                    if (MODE == TE_COLLISIONS) coef_all[(int64_t)b * out_cap + slot] = coefs[cid];
+                }
+                ++slot;
+            }
+        }
+    }
+}
+
+enum class Narrow { SAT, PROXIMITY, CCD_COLLECT, CCD_IMPULSE, MINCOL, TRI_EDGE };
 
 struct Query {
     ccc_context *ctx;
@@ -659,10 +876,25 @@ struct Query {
     void *out_roots;        // CCD_COLLECT [B,cap]
     void *dv, *dx;          // CCD_IMPULSE
     int32_t *cnt;
+    int te_mode;            // TRI_EDGE: TE_COLLISIONS / TE_CANDIDATES / TE_CANDIDATES2
+    const int32_t *mesh_end;  // packed meshes (B = 1): per leaf position, end of its mesh
+    int32_t *masks;         // TRI_EDGE candidates2 [B,cap,2]
     ccc_counts *counts;
     uint32_t flags;
     hipStream_t s;
 };
+
+// End of a pair chunk: its sub-primitive items are done; the next chunk refills the list.
+__global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
+    for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
+        ctr[b].items_done += ctr[b].items;
+        ctr[b].items = 0;
+    }
+}
+
+// Pairs per narrow-phase chunk: the item list holds at most 15 items per pair of a chunk,
+// 4 GiB per batch element at this size.
+constexpr int64_t kPairChunk = (int64_t)1 << 25;
 
 static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B) {
     int64_t want = (items + kThreads - 1) / kThreads;
@@ -694,6 +926,14 @@ static ccc_status run_query(const Query &q) {
     // outputs start at -1 (index rows, roots) or 0 (impulses)
     switch (q.narrow) {
         case Narrow::SAT: st = fill_u64(q.out_rows, B * q.out_cap * 2, ~0ull, s); break;
+        case Narrow::TRI_EDGE:
+            st = fill_u64(q.out_rows, B * q.out_cap * (q.te_mode == TE_CANDIDATES2 ? 2 : 4), ~0ull, s);
+            if (st == CCC_OK && q.te_mode == TE_COLLISIONS) {
+                if (sizeof(T) == 8) st = fill_u64(q.out_roots, B * q.out_cap, 0xBFF0000000000000ull, s);
+                else st = fill_u32(q.out_roots, B * q.out_cap, 0xBF800000u, s);
+            }
+            if (st == CCC_OK && q.te_mode == TE_CANDIDATES2) st = fill_u32(q.masks, B * q.out_cap * 2, 0u, s);
+            break;
         case Narrow::PROXIMITY:
         case Narrow::MINCOL: st = fill_u64(q.out_rows, B * q.out_cap * 3, ~0ull, s); break;
         case Narrow::CCD_COLLECT:
@@ -730,10 +970,10 @@ static ccc_status run_query(const Query &q) {
         tm.begin("traverse");
         if (q.to_check)
             hipLaunchKernelGGL((k_traverse<T, true>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
+                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr);
         else
             hipLaunchKernelGGL((k_traverse<T, false>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, pairs, capK, ctr);
+                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr);
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
@@ -763,38 +1003,58 @@ static ccc_status run_query(const Query &q) {
         CCC_TIMED(tm, "sat", hipLaunchKernelGGL((k_sat<T>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n, thr,
                                                 q.out_rows, q.out_cap, ctr));
         CCC_LAUNCH_CHECK("k_sat");
+    } else if (q.narrow == Narrow::TRI_EDGE) {
+        tm.begin("tri_edge");
+        if (q.te_mode == TE_COLLISIONS)
+            hipLaunchKernelGGL((k_tri_edge<T, TE_COLLISIONS>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n,
+                               q.out_rows, (T *)q.out_roots, (int32_t *)nullptr, q.out_cap, ctr);
+        else if (q.te_mode == TE_CANDIDATES)
+            hipLaunchKernelGGL((k_tri_edge<T, TE_CANDIDATES>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n,
+                               q.out_rows, (T *)nullptr, (int32_t *)nullptr, q.out_cap, ctr);
+        else
+            hipLaunchKernelGGL((k_tri_edge<T, TE_CANDIDATES2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, n,
+                               q.out_rows, (T *)nullptr, q.masks, q.out_cap, ctr);
+        tm.end();
+        CCC_LAUNCH_CHECK("k_tri_edge");
     } else if (q.narrow == Narrow::MINCOL) {
-        CCC_TIMED(tm, "mincol", hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0,
-                                                   (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr));
+        CCC_TIMED(tm, "mincol", hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (int64_t)0,
+                                                   capK, (const T *)q.t0, (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr));
         CCC_LAUNCH_CHECK("k_subprim");
     } else {
-        const int64_t capM = std::max<int64_t>(15 * kmax, 1);
+        // sub-primitive items and the narrow phase run over chunks of the pair list
+        const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, kPairChunk), 1);
+        const int64_t capM = 15 * chunk;
         int64_t *items = ws.get_as<int64_t>(B * capM);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
-        tm.begin("subprim");
-        if (q.narrow == Narrow::PROXIMITY)
-            hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)nullptr,
-                               n, thr, items, capM, ctr);
-        else
-            hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (const T *)q.t0, (const T *)q.t1,
-                               n, thr, items, capM, ctr);
-        tm.end();
-        CCC_LAUNCH_CHECK("k_subprim");
+        const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
-        tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
-        if (q.narrow == Narrow::PROXIMITY)
-            hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr, q.out_rows,
-                               q.out_cap, ctr);
-        else if (q.narrow == Narrow::CCD_COLLECT)
-            hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                               (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap, (T *)nullptr,
-                               (T *)nullptr, (int *)nullptr, ctr);
-        else
-            hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                               (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0, (T *)q.dv,
-                               (T *)q.dx, q.cnt, ctr);
-        tm.end();
-        CCC_LAUNCH_CHECK("narrow phase");
+        for (int64_t c0 = 0; c0 < kmax; c0 += chunk) {
+            const int64_t c1 = c0 + chunk;
+            tm.begin("subprim");
+            if (q.narrow == Narrow::PROXIMITY)
+                hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
+                                   (const T *)nullptr, n, thr, items, capM, ctr);
+            else
+                hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
+                                   (const T *)q.t1, n, thr, items, capM, ctr);
+            tm.end();
+            CCC_LAUNCH_CHECK("k_subprim");
+            tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
+            if (q.narrow == Narrow::PROXIMITY)
+                hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
+                                   q.out_rows, q.out_cap, ctr);
+            else if (q.narrow == Narrow::CCD_COLLECT)
+                hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                                   (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap, (T *)nullptr,
+                                   (T *)nullptr, (int *)nullptr, ctr);
+            else
+                hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                                   (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
+                                   (T *)q.dv, (T *)q.dx, q.cnt, ctr);
+            tm.end();
+            hipLaunchKernelGGL(k_fold_items, dim3(1), dim3(64), 0, s, ctr, B);
+            CCC_LAUNCH_CHECK("narrow phase");
+        }
     }
     CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
     CCC_HIP_TRY(hipStreamSynchronize(s));
@@ -803,7 +1063,7 @@ static ccc_status run_query(const Query &q) {
         const int64_t K = std::min<int64_t>((int64_t)h[b].cand, capK);
         if (q.counts) {
             q.counts[b].candidates = (int64_t)h[b].cand;
-            q.counts[b].subprims = (int64_t)h[b].items;
+            q.counts[b].subprims = (int64_t)(q.narrow == Narrow::MINCOL ? h[b].items : h[b].items_done);
             q.counts[b].roots = (int64_t)h[b].roots;
             q.counts[b].collisions = q.narrow == Narrow::MINCOL ? (int64_t)h[b].items : (int64_t)h[b].coll;
         }
@@ -829,6 +1089,8 @@ static ccc_status run(const Query &q) {
     if (q.narrow == Narrow::CCD_IMPULSE && (!q.masses || !q.dv || !q.dx || !q.cnt)) return bad_arg("NULL impulse tensor");
     if (q.narrow != Narrow::CCD_IMPULSE && (!q.out_rows || q.out_cap < 0)) return bad_arg("NULL output tensor");
     if (q.narrow == Narrow::CCD_COLLECT && !q.out_roots) return bad_arg("NULL roots tensor");
+    if (q.narrow == Narrow::TRI_EDGE && q.te_mode == TE_COLLISIONS && !q.out_roots) return bad_arg("NULL coefficient tensor");
+    if (q.narrow == Narrow::TRI_EDGE && q.te_mode == TE_CANDIDATES2 && !q.masks) return bad_arg("NULL mask tensor");
     CCC_HIP_TRY(hipSetDevice(q.ctx->device));
     if (q.dtype == CCC_FLOAT64) return run_query<double>(q);
     if (q.dtype == CCC_FLOAT32) return run_query<float>(q);
@@ -856,6 +1118,75 @@ static ccc_status from_collisions(ccc_context *ctx, const int64_t *coll, const v
     return CCC_OK;
 }
 
+// mesh_end[i] = offsets[m + 1] for the mesh m that owns triangle i (offsets ascending).
+__global__ void k_mesh_end(const int64_t *__restrict__ offsets, int64_t meshes, int64_t total, int32_t *__restrict__ mesh_end) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int64_t lo = 0, hi = meshes - 1;      // first m with offsets[m + 1] > i
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (offsets[mid + 1] > i) hi = mid;
+        else lo = mid + 1;
+    }
+    mesh_end[i] = (int32_t)offsets[lo + 1];
+}
+
+// Independent meshes of different sizes stored back to back, handled as one soup: one BVH
+// over the concatenation (compat Morton codes are all zero, so leaf position = triangle
+// index and every subtree covers a contiguous index range) and one traversal whose queries
+// never enter index ranges of later meshes.  Every stage after the traversal works on the
+// resulting pairs unchanged.  The pair set of each mesh is exactly its own single-mesh set.
+static ccc_status packed_impulses(ccc_context *ctx, const void *t0, const void *t1, const void *masses, const int64_t *offsets,
+                                  int64_t meshes, int32_t dtype, int32_t max_candidates, void *dv, void *dx, int32_t *cnt,
+                                  ccc_counts *counts, uint32_t flags, hipStream_t s) {
+    clear_error();
+    if (!ctx) return bad_arg("context is NULL");
+    if (meshes < 0 || (meshes > 0 && !offsets)) return bad_arg("offsets is NULL");
+    if (meshes == 0) return CCC_OK;
+    if (offsets[0] != 0) return bad_arg("offsets[0] must be 0");
+    for (int64_t m = 0; m < meshes; ++m)
+        if (offsets[m + 1] <= offsets[m]) return bad_arg("every mesh needs at least one triangle (offsets strictly increasing)");
+    const int64_t n = offsets[meshes];
+    if (n >= ((int64_t)1 << 28)) return bad_arg("at most 2^28 - 1 triangles in a packed batch");
+    if (!t0 || !t1 || !masses || !dv || !dx || !cnt) return bad_arg("NULL tensor pointer");
+    if (max_candidates < 1) return bad_arg("max_candidates must be >= 1");
+    if (dtype != CCC_FLOAT32 && dtype != CCC_FLOAT64) return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+    CCC_HIP_TRY(hipSetDevice(ctx->device));
+    const size_t esz = dtype == CCC_FLOAT64 ? 8 : 4;
+    Workspace ws(ctx, s);
+    void *bboxes = ws.get((size_t)(2 * n - 1) * 6 * esz);
+    int64_t *tree = ws.get_as<int64_t>((2 * n - 1) * 4);
+    int32_t *mesh_end = ws.get_as<int32_t>(n);
+    int64_t *d_off = ws.get_as<int64_t>(meshes + 1);
+    if (ws.failed()) return CCC_ERR_NO_MEMORY;
+    CCC_HIP_TRY(hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * (meshes + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mesh_end, dim3(grid_for(n, 256)), dim3(256), 0, s, d_off, meshes, n, mesh_end);
+    CCC_LAUNCH_CHECK("k_mesh_end");
+    ccc_status st = build_bvh_device(ctx, t0, t1, 1, n, dtype, bboxes, tree, s);
+    if ((st & CCC_ERR_MASK) != CCC_OK) return st;
+    Query q{};
+    q.ctx = ctx;
+    q.bboxes = bboxes;
+    q.tree = tree;
+    q.t0 = t0;
+    q.t1 = t1;
+    q.masses = masses;
+    q.B = 1;
+    q.n = n;
+    q.dtype = dtype;
+    q.flags = flags;
+    q.s = s;
+    q.cap_candidates = (int64_t)max_candidates * n;
+    q.narrow = Narrow::CCD_IMPULSE;
+    q.dv = dv;
+    q.dx = dx;
+    q.cnt = cnt;
+    q.counts = counts;
+    q.mesh_end = mesh_end;
+    if (dtype == CCC_FLOAT64) return run_query<double>(q);
+    return run_query<float>(q);
+}
+
 }  // namespace ccc
 
 using ccc::Narrow;
@@ -876,6 +1207,23 @@ static Query base_query(ccc_context *ctx, const void *bboxes, const int64_t *tre
     return q;
 }
 
+static ccc_status tri_edge_query(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                 int64_t batch, int64_t n, int32_t dtype, int32_t max_collisions, int mode, int64_t *out,
+                                 void *coefs, int32_t *masks, ccc_counts *counts, uint32_t flags, void *stream) {
+    Query q = base_query(ctx, bboxes, tree, triangles, batch, n, dtype, flags, stream);
+    q.thr = 0.0;
+    q.cap_candidates = (int64_t)max_collisions * n;
+    q.out_cap = (int64_t)max_collisions * n;
+    q.narrow = Narrow::TRI_EDGE;
+    q.te_mode = mode;
+    q.out_rows = out;
+    q.out_roots = coefs;
+    q.masks = masks;
+    q.counts = counts;
+    if (max_collisions < 1) return ccc::bad_arg("max_collisions must be >= 1");
+    return ccc::run(q);
+}
+
 extern "C" {
 
 ccc_status ccc_find_collisions(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles, int64_t batch,
@@ -890,6 +1238,34 @@ ccc_status ccc_find_collisions(ccc_context *ctx, const void *bboxes, const int64
     q.counts = counts;
     if (max_collisions < 1) return ccc::bad_arg("max_collisions must be >= 1");
     return ccc::run(q);
+}
+
+ccc_status ccc_find_triangle_edge_collisions(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                             int64_t batch, int64_t n, int32_t dtype, int32_t max_collisions, int64_t *out,
+                                             void *coefs, ccc_counts *counts, uint32_t flags, void *stream) {
+    return tri_edge_query(ctx, bboxes, tree, triangles, batch, n, dtype, max_collisions, ccc::TE_COLLISIONS, out, coefs, nullptr,
+                          counts, flags, stream);
+}
+
+ccc_status ccc_find_triangle_edge_candidates(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                             int64_t batch, int64_t n, int32_t dtype, int32_t max_collisions, int64_t *out,
+                                             ccc_counts *counts, uint32_t flags, void *stream) {
+    return tri_edge_query(ctx, bboxes, tree, triangles, batch, n, dtype, max_collisions, ccc::TE_CANDIDATES, out, nullptr, nullptr,
+                          counts, flags, stream);
+}
+
+ccc_status ccc_find_triangle_edge_candidates2(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                              int64_t batch, int64_t n, int32_t dtype, int32_t max_collisions, int64_t *pairs,
+                                              int32_t *masks, ccc_counts *counts, uint32_t flags, void *stream) {
+    return tri_edge_query(ctx, bboxes, tree, triangles, batch, n, dtype, max_collisions, ccc::TE_CANDIDATES2, pairs, nullptr, masks,
+                          counts, flags, stream);
+}
+
+ccc_status ccc_compute_impulses_packed(ccc_context *ctx, const void *triangles, const void *triangles_next, const void *masses,
+                                       const int64_t *offsets, int64_t num_meshes, int32_t dtype, int32_t max_candidates,
+                                       void *dv, void *dx, int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream) {
+    return ccc::packed_impulses(ctx, triangles, triangles_next, masses, offsets, num_meshes, dtype, max_candidates, dv, dx, cnt,
+                                counts, flags, (hipStream_t)stream);
 }
 
 ccc_status ccc_find_proximity(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles, int64_t batch,

@@ -94,6 +94,10 @@ void finish(ccc_status st, const char *what) {
         warn("Number of actual collisions exceeds maximal allowed number. Some of the collisions are omitted.");
 }
 
+// The native calls run without the GIL (they touch no Python object), so that host threads
+// driving different HIP streams -- a serving loop's worker slots -- overlap on the device.
+#define CCC_NOGIL(expr) ([&]() { py::gil_scoped_release nogil_; return (expr); }())
+
 ccc_counts *counts_buffer(int64_t B) {
     g_counts.assign((size_t)B, ccc_counts{0, 0, 0, 0});
     return g_counts.data();
@@ -109,8 +113,8 @@ std::vector<torch::Tensor> bvh_forward(at::Tensor triangles) {
     TORCH_CHECK(n >= 1, "triangles must hold at least one triangle");
     auto bboxes = at::empty({B, 2 * n - 1, 6}, triangles.options());
     auto tree = at::empty({B, 2 * n - 1, 4}, triangles.options().dtype(at::kLong));
-    finish(ccc_bvh(context_for(triangles), triangles.data_ptr(), B, n, dtype_of(triangles), bboxes.data_ptr(),
-                   tree.data_ptr<int64_t>(), stream_for(triangles)),
+    finish(CCC_NOGIL(ccc_bvh(context_for(triangles), triangles.data_ptr(), B, n, dtype_of(triangles), bboxes.data_ptr(),
+                   tree.data_ptr<int64_t>(), stream_for(triangles))),
            "bvh");
     return {bboxes, tree};
 }
@@ -126,8 +130,8 @@ std::vector<torch::Tensor> bvh_forward_motion(at::Tensor triangles, at::Tensor t
     check_like(triangles_next, triangles, "triangles_next");
     auto bboxes = at::empty({B, 2 * n - 1, 6}, triangles.options());
     auto tree = at::empty({B, 2 * n - 1, 4}, triangles.options().dtype(at::kLong));
-    finish(ccc_bvh_motion(context_for(triangles), triangles.data_ptr(), triangles_next.data_ptr(), B, n, dtype_of(triangles),
-                          bboxes.data_ptr(), tree.data_ptr<int64_t>(), stream_for(triangles)),
+    finish(CCC_NOGIL(ccc_bvh_motion(context_for(triangles), triangles.data_ptr(), triangles_next.data_ptr(), B, n, dtype_of(triangles),
+                          bboxes.data_ptr(), tree.data_ptr<int64_t>(), stream_for(triangles))),
            "bvh_motion");
     return {bboxes, tree};
 }
@@ -143,26 +147,71 @@ at::Tensor find_collisions(at::Tensor bboxes, at::Tensor tree, at::Tensor triang
     check_soup(triangles, B, n, "triangles");
     TORCH_CHECK(max_collisions >= 1, "max_collisions must be >= 1");
     auto out = at::empty({B, n * max_collisions, 2}, bboxes.options().dtype(at::kLong));
-    finish(ccc_find_collisions(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
+    finish(CCC_NOGIL(ccc_find_collisions(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
                                dtype_of(triangles), threshold, max_collisions, out.data_ptr<int64_t>(), counts_buffer(B),
-                               g_flags, stream_for(triangles)),
+                               g_flags, stream_for(triangles))),
            "find_collisions");
     return out;
 }
 
-[[noreturn]] void not_in_this_build(const char *name) {
-    TORCH_CHECK_NOT_IMPLEMENTED(false, "cccollisions.", name,
-                                " (triangle-edge intersection queries) is not implemented by cccollisions_amd yet");
+// Triangle-edge queries (src/cccollisions.cpp:151-218).  `threshold` is accepted and unused,
+// as in the reference: traversal and SAT run at threshold 0.
+struct TeArgs {
+    int64_t B, n;
+};
+
+static TeArgs te_check(const at::Tensor &bboxes, const at::Tensor &tree, const at::Tensor &triangles, int max_collisions) {
+    CHECK_INPUT(bboxes);
+    CHECK_INPUT(tree);
+    CHECK_INPUT(triangles);
+    check_tree(bboxes, tree, triangles);
+    const int64_t B = bboxes.size(0), n = (bboxes.size(1) + 1) / 2;
+    check_soup(triangles, B, n, "triangles");
+    TORCH_CHECK(max_collisions >= 1, "max_collisions must be >= 1");
+    return {B, n};
 }
 
-std::vector<torch::Tensor> find_triangle_edge_collisions(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
-    not_in_this_build("find_triangle_edge_collisions");
+std::vector<torch::Tensor> find_triangle_edge_collisions(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles,
+                                                         float threshold = 0., int max_collisions = 16) {
+    (void)threshold;
+    const TeArgs a = te_check(bboxes, tree, triangles, max_collisions);
+    const at::DeviceGuard guard(triangles.device());
+    auto out = at::empty({a.B, a.n * max_collisions, 4}, bboxes.options().dtype(at::kLong));
+    auto coefs = at::empty({a.B, a.n * max_collisions, 1}, bboxes.options());
+    finish(CCC_NOGIL(ccc_find_triangle_edge_collisions(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                             triangles.data_ptr(), a.B, a.n, dtype_of(triangles), max_collisions,
+                                             out.data_ptr<int64_t>(), coefs.data_ptr(), counts_buffer(a.B), g_flags,
+                                             stream_for(triangles))),
+           "find_triangle_edge_collisions");
+    return {out, coefs};
 }
-torch::Tensor find_triangle_edge_candidates(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
-    not_in_this_build("find_triangle_edge_candidates");
+
+torch::Tensor find_triangle_edge_candidates(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, float threshold = 0.,
+                                            int max_collisions = 16) {
+    (void)threshold;
+    const TeArgs a = te_check(bboxes, tree, triangles, max_collisions);
+    const at::DeviceGuard guard(triangles.device());
+    auto out = at::empty({a.B, a.n * max_collisions, 4}, bboxes.options().dtype(at::kLong));
+    finish(CCC_NOGIL(ccc_find_triangle_edge_candidates(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                             triangles.data_ptr(), a.B, a.n, dtype_of(triangles), max_collisions,
+                                             out.data_ptr<int64_t>(), counts_buffer(a.B), g_flags, stream_for(triangles))),
+           "find_triangle_edge_candidates");
+    return out;
 }
-std::vector<torch::Tensor> find_triangle_edge_candidates2(at::Tensor, at::Tensor, at::Tensor, float = 0., int = 16) {
-    not_in_this_build("find_triangle_edge_candidates2");
+
+std::vector<torch::Tensor> find_triangle_edge_candidates2(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles,
+                                                          float threshold = 0., int max_collisions = 16) {
+    (void)threshold;
+    const TeArgs a = te_check(bboxes, tree, triangles, max_collisions);
+    const at::DeviceGuard guard(triangles.device());
+    auto pairs = at::empty({a.B, a.n * max_collisions, 2}, bboxes.options().dtype(at::kLong));
+    auto masks = at::empty({a.B, a.n * max_collisions, 2}, bboxes.options().dtype(at::kInt));
+    finish(CCC_NOGIL(ccc_find_triangle_edge_candidates2(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+                                              triangles.data_ptr(), a.B, a.n, dtype_of(triangles), max_collisions,
+                                              pairs.data_ptr<int64_t>(), masks.data_ptr<int32_t>(), counts_buffer(a.B),
+                                              g_flags, stream_for(triangles))),
+           "find_triangle_edge_candidates2");
+    return {pairs, masks};
 }
 
 std::vector<torch::Tensor> continuous(at::Tensor bboxes, at::Tensor tree, at::Tensor triangles, at::Tensor triangles_next,
@@ -187,7 +236,7 @@ std::vector<torch::Tensor> continuous(at::Tensor bboxes, at::Tensor tree, at::Te
     }
     auto coll = at::empty({B, n * max_collisions_per_triangle, 3}, bboxes.options().dtype(at::kLong));
     auto roots = at::empty({B, n * max_collisions_per_triangle, 1}, bboxes.options());
-    ccc_status st = tc ? ccc_find_collisions_continuous_partial(
+    ccc_status st = CCC_NOGIL(tc ? ccc_find_collisions_continuous_partial(
                              context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(),
                              triangles_next.data_ptr(), tc, B, n, dtype_of(triangles), max_candidates_per_triangle,
                              max_collisions_per_triangle, coll.data_ptr<int64_t>(), roots.data_ptr(), counts_buffer(B), g_flags,
@@ -196,7 +245,7 @@ std::vector<torch::Tensor> continuous(at::Tensor bboxes, at::Tensor tree, at::Te
                                                         triangles.data_ptr(), triangles_next.data_ptr(), B, n, dtype_of(triangles),
                                                         max_candidates_per_triangle, max_collisions_per_triangle,
                                                         coll.data_ptr<int64_t>(), roots.data_ptr(), counts_buffer(B), g_flags,
-                                                        stream_for(triangles));
+                                                        stream_for(triangles)));
     finish(st, name);
     return {coll, roots};
 }
@@ -227,9 +276,9 @@ std::vector<torch::Tensor> find_proximity(at::Tensor bboxes, at::Tensor tree, at
     check_soup(triangles, B, n, "triangles");
     TORCH_CHECK(max_candidates_per_triangle >= 1, "max_candidates_per_triangle must be >= 1");
     auto out = at::empty({B, n * max_candidates_per_triangle, 3}, bboxes.options().dtype(at::kLong));
-    finish(ccc_find_proximity(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
+    finish(CCC_NOGIL(ccc_find_proximity(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(), B, n,
                               dtype_of(triangles), threshold, max_candidates_per_triangle, out.data_ptr<int64_t>(),
-                              counts_buffer(B), g_flags, stream_for(triangles)),
+                              counts_buffer(B), g_flags, stream_for(triangles))),
            "find_proximity");
     return {out};
 }
@@ -248,9 +297,9 @@ torch::Tensor find_mincol_cand(at::Tensor bboxes, at::Tensor tree, at::Tensor tr
     check_like(triangles_next, triangles, "triangles_next");
     TORCH_CHECK(max_collisions >= 1, "max_collisions must be >= 1");
     auto out = at::empty({B, n * max_collisions * 15, 3}, bboxes.options().dtype(at::kLong));
-    finish(ccc_find_mincol_cand(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(),
+    finish(CCC_NOGIL(ccc_find_mincol_cand(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(), triangles.data_ptr(),
                                 triangles_next.data_ptr(), B, n, dtype_of(triangles), max_collisions, out.data_ptr<int64_t>(),
-                                counts_buffer(B), g_flags, stream_for(triangles)),
+                                counts_buffer(B), g_flags, stream_for(triangles))),
            "find_mincol_cand");
     return out;
 }
@@ -281,7 +330,7 @@ std::vector<torch::Tensor> impulses(at::Tensor bboxes, at::Tensor tree, at::Tens
     auto dv = at::empty({B, n, 3, 3}, bboxes.options());
     auto dx = at::empty({B, n, 3, 3}, bboxes.options());
     auto cnt = at::empty({B, n, 3}, bboxes.options().dtype(at::kInt));
-    ccc_status st = tc ? ccc_compute_impulses_partial(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
+    ccc_status st = CCC_NOGIL(tc ? ccc_compute_impulses_partial(context_for(triangles), bboxes.data_ptr(), tree.data_ptr<int64_t>(),
                                                       triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(),
                                                       tc, B, n, dtype_of(triangles), max_candidates_per_triangle,
                                                       max_collisions_per_triangle, dv.data_ptr(), dx.data_ptr(),
@@ -290,8 +339,39 @@ std::vector<torch::Tensor> impulses(at::Tensor bboxes, at::Tensor tree, at::Tens
                                               triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(), B, n,
                                               dtype_of(triangles), max_candidates_per_triangle, max_collisions_per_triangle,
                                               dv.data_ptr(), dx.data_ptr(), cnt.data_ptr<int32_t>(), counts_buffer(B), g_flags,
-                                              stream_for(triangles));
+                                              stream_for(triangles)));
     finish(st, name);
+    return {dv, dx, cnt};
+}
+
+// Meshes of different sizes back to back ([N, 3, 3] soups, [N, 3] masses, offsets[m] = first
+// triangle of mesh m, offsets[-1] = N): every mesh's compute_impulses after its own
+// bvh_motion, in one pass of launches.
+std::vector<torch::Tensor> compute_impulses_packed(at::Tensor triangles, at::Tensor triangles_next, at::Tensor triangles_mass,
+                                                   std::vector<int64_t> offsets, int max_candidates_per_triangle = 16) {
+    CHECK_INPUT(triangles);
+    CHECK_INPUT(triangles_next);
+    CHECK_INPUT(triangles_mass);
+    TORCH_CHECK(triangles.dim() == 3 && triangles.size(1) == 3 && triangles.size(2) == 3, "triangles must have shape [N, 3, 3]");
+    TORCH_CHECK(triangles.scalar_type() == at::kDouble || triangles.scalar_type() == at::kFloat,
+                "triangles must be float32 or float64");
+    check_like(triangles_next, triangles, "triangles_next");
+    check_like(triangles_mass, triangles, "triangles_mass");
+    const int64_t n = triangles.size(0);
+    TORCH_CHECK(triangles_next.sizes() == triangles.sizes(), "triangles_next must have the shape of triangles");
+    TORCH_CHECK(triangles_mass.numel() == n * 3, "triangles_mass must have shape [N, 3]");
+    TORCH_CHECK(offsets.size() >= 2 && offsets.front() == 0 && offsets.back() == n,
+                "offsets must start at 0 and end at the number of triangles");
+    TORCH_CHECK(max_candidates_per_triangle >= 1, "max_candidates_per_triangle must be >= 1");
+    const at::DeviceGuard guard(triangles.device());
+    auto dv = at::empty({n, 3, 3}, triangles.options());
+    auto dx = at::empty({n, 3, 3}, triangles.options());
+    auto cnt = at::empty({n, 3}, triangles.options().dtype(at::kInt));
+    finish(CCC_NOGIL(ccc_compute_impulses_packed(context_for(triangles), triangles.data_ptr(), triangles_next.data_ptr(),
+                                       triangles_mass.data_ptr(), offsets.data(), (int64_t)offsets.size() - 1,
+                                       dtype_of(triangles), max_candidates_per_triangle, dv.data_ptr(), dx.data_ptr(),
+                                       cnt.data_ptr<int32_t>(), counts_buffer(1), g_flags, stream_for(triangles))),
+           "compute_impulses_packed");
     return {dv, dx, cnt};
 }
 
@@ -332,10 +412,10 @@ std::vector<torch::Tensor> compute_impulses_from_collisions(at::Tensor collision
     auto dv = at::empty({B, n, 3, 3}, triangles.options());
     auto dx = at::empty({B, n, 3, 3}, triangles.options());
     auto cnt = at::empty({B, n, 3}, triangles.options().dtype(at::kInt));
-    finish(ccc_compute_impulses_from_collisions(context_for(triangles), collisions.data_ptr<int64_t>(), roots.data_ptr(), m,
+    finish(CCC_NOGIL(ccc_compute_impulses_from_collisions(context_for(triangles), collisions.data_ptr<int64_t>(), roots.data_ptr(), m,
                                                 triangles.data_ptr(), triangles_next.data_ptr(), triangles_masses.data_ptr(), B,
                                                 n, dtype_of(triangles), dv.data_ptr(), dx.data_ptr(), cnt.data_ptr<int32_t>(),
-                                                g_flags, stream_for(triangles)),
+                                                g_flags, stream_for(triangles))),
            "compute_impulses_from_collisions");
     return {dv, dx, cnt};
 }
@@ -393,6 +473,10 @@ PYBIND11_MODULE(_cccollisions, m) {
           py::arg("triangles"), py::arg("threshold") = 0., py::arg("max_collisions") = 16);
     m.def("find_triangle_edge_candidates2", &find_triangle_edge_candidates2, "", py::arg("bboxes"), py::arg("tree"),
           py::arg("triangles"), py::arg("threshold") = 0., py::arg("max_collisions") = 16);
+    m.def("compute_impulses_packed", &compute_impulses_packed,
+          "compute_impulses for meshes of different sizes stored back to back (one BVH and one traversal over all)",
+          py::arg("triangles"), py::arg("triangles_next"), py::arg("triangles_mass"), py::arg("offsets"),
+          py::arg("max_candidates_per_triangle") = 16);
     m.def("find_collisions_continuous", &find_collisions_continuous, "", py::arg("bboxes"), py::arg("tree"), py::arg("triangles"),
           py::arg("triangles_next"), py::arg("max_candidates_per_triangle") = 16, py::arg("max_collisions_per_triangle") = 16);
     m.def("find_collisions_continuous_partial", &find_collisions_continuous_partial, "", py::arg("bboxes"), py::arg("tree"),

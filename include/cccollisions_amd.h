@@ -132,6 +132,40 @@ ccc_status ccc_find_collisions_continuous_partial(ccc_context *ctx, const void *
                                                   int64_t *collisions, void *roots,
                                                   ccc_counts *counts, uint32_t flags, void *stream);
 
+/* Replaces find_triangle_edge_collisions_from_bbox_tree -- src/collisions_static.cu:612-779
+ * (binding src/cccollisions.cpp:151-173; kernels :345-500).  Traversal and SAT at threshold 0
+ * (the binding's threshold is unused), pairs sharing a vertex rejected, then each of the six
+ * edges of a pair tested against the other triangle.  out: [batch, n*max_collisions, 4]
+ * int64 rows (x, y, edge id 0..5, loop id -1/0), -1 padded; coefs: [batch, n*max_collisions]
+ * of dtype, the edge parameter of the piercing point, -1 padded.  Rows of batch element b
+ * start at b * n*max_collisions (the reference's copy uses a stride of 3 there and writes
+ * up to 6 rows per pair past the end; here rows beyond the capacity are dropped with
+ * CCC_WARN_TRUNCATED). */
+ccc_status ccc_find_triangle_edge_collisions(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                             const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                                             int32_t max_collisions, int64_t *out, void *coefs,
+                                             ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Replaces find_triangle_edge_candidates -- src/collisions_static.cu:782-943 (binding
+ * src/cccollisions.cpp:176-194; kernel :520-604).  Pairs may share one vertex; a row
+ * (x, y, edge id, loop id) for every edge that does not end on a corner of the other
+ * triangle, loop id 0 when it starts on one (edge ids 0..2 only).  out: [batch,
+ * n*max_collisions, 4] int64, -1 padded, truncated as above. */
+ccc_status ccc_find_triangle_edge_candidates(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                             const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                                             int32_t max_collisions, int64_t *out, ccc_counts *counts,
+                                             uint32_t flags, void *stream);
+
+/* Replaces find_triangle_edge_candidates2 -- src/icontour.cu:110-282 (binding
+ * src/cccollisions.cpp:197-218; kernel icontour.cu:11-108).  One row per pair that passes
+ * the candidate filter: pairs [batch, n*max_collisions, 2] int64 (-1 padded) and masks
+ * [batch, n*max_collisions, 2] int32 (0 padded) with bit e of mask.x set for each edge
+ * row the candidates query would emit and bit e of mask.y for each loop node. */
+ccc_status ccc_find_triangle_edge_candidates2(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                              const void *triangles, int64_t batch, int64_t n, int32_t dtype,
+                                              int32_t max_collisions, int64_t *pairs, int32_t *masks,
+                                              ccc_counts *counts, uint32_t flags, void *stream);
+
 /* Replaces find_minimal_candidates_from_bbox_tree -- src/collisions_continuous.cu:1379-1519
  * (binding src/cccollisions.cpp:405-423).  out: [batch, n*max_collisions*15, 3] int64. */
 ccc_status ccc_find_mincol_cand(ccc_context *ctx, const void *bboxes, const int64_t *tree,
@@ -148,6 +182,24 @@ ccc_status ccc_compute_impulses(ccc_context *ctx, const void *bboxes, const int6
                                 const void *masses, int64_t batch, int64_t n, int32_t dtype,
                                 int32_t max_candidates, int32_t max_collisions, void *dv, void *dx,
                                 int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream);
+
+/* Packed batch of independent meshes of different sizes: compute_impulses for every mesh
+ * in one pass of launches.  No single reference function corresponds: the reference loops
+ * over the equal-size meshes of a [batch, n] tensor one at a time
+ * (src/collisions_impulses.cu:1069-1397, `for (int bidx = 0; ...)`); this entry point gives
+ * each mesh the results of that function (bvh_motion, src/build_bvh.cu:570-648, followed by
+ * collision_impulses_from_bbox_tree) for meshes of any sizes.
+ * triangles, triangles_next: [N, 3, 3], masses: [N, 3], meshes back to back, mesh m owning
+ * triangles [offsets[m], offsets[m+1]) (offsets: HOST array of num_meshes + 1 entries,
+ * offsets[0] = 0, strictly increasing, N = offsets[num_meshes] < 2^28).  dv, dx: [N, 3, 3],
+ * cnt: [N, 3].  max_candidates bounds the candidate list at max_candidates * N pairs over
+ * the whole batch (grown as in ccc_compute_impulses).  counts: one entry, batch totals.
+ * Pairs never cross meshes; each mesh's pairs, roots and counts equal its single-mesh call,
+ * its impulses within fp64 atomic reordering. */
+ccc_status ccc_compute_impulses_packed(ccc_context *ctx, const void *triangles, const void *triangles_next,
+                                       const void *masses, const int64_t *offsets, int64_t num_meshes,
+                                       int32_t dtype, int32_t max_candidates, void *dv, void *dx, int32_t *cnt,
+                                       ccc_counts *counts, uint32_t flags, void *stream);
 
 /* Replaces collision_impulses_from_bbox_tree_partial -- src/collisions_impulses.cu:1402-1734
  * (binding src/cccollisions.cpp:336-370). */

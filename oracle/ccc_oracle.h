@@ -64,6 +64,9 @@ typedef struct {
     int64_t orc_static_collisions##SFX(const REAL *tris, int64_t n, const int64_t *pairs,   \
                                        int64_t k, REAL threshold, int64_t *out,             \
                                        int64_t cap);                                        \
+    int64_t orc_triangle_edge##SFX(const REAL *tris, int64_t n, const int64_t *pairs,       \
+                                   int64_t k, int mode, int64_t *rows, REAL *coefs,         \
+                                   int32_t *masks, int64_t cap);                            \
     int64_t orc_mincol##SFX(const REAL *tris, const REAL *tris_next, int64_t n,             \
                             const int64_t *pairs, int64_t k, int64_t *out, int64_t cap);    \
     int orc_cubic_coeffs##SFX(const REAL *tet, REAL *coef);                                 \

@@ -23,6 +23,9 @@
  *   orc_static_collisions - textbook SAT cases (separated, touching, interpenetrating,
  *                        shared vertex) and a numpy SAT re-implementation.
  *   orc_mincol         - numpy re-implementation of the 15-way swept-box filter.
+ *   orc_triangle_edge  - hand cases (edge piercing a face at a known parameter, parallel,
+ *                        outside, shared vertex loops) and a numpy re-implementation of the
+ *                        edge-plane intersection on random soups.
  *   ULP-level behaviour of CUDA libm (cbrt/acos/cos/sin/rsqrt) and the exact rounding of
  *   CAMPARY certifiedMul are parity unpinned (no reference binary or vectors exist); they
  *   are covered by the 1e-9 relative tolerance.
@@ -877,6 +880,101 @@ int64_t FN(orc_static_collisions)(const REAL *tris, int64_t n, const int64_t *pa
         int shared = FN(count_shared)(a.v, b.v);
         if (hit && !(shared > 0)) {
             if (R < cap) { out[2 * R] = x; out[2 * R + 1] = y; }
+            R++;
+        }
+    }
+    return R;
+}
+
+/* figures.cu:605-610 hasVertex(tri, v): exact coordinate match with one of the corners */
+static inline int FN(has_vertex)(TRIT t, V3T v) {
+    return FN(veq)(t.v[0], v) || FN(veq)(t.v[1], v) || FN(veq)(t.v[2], v);
+}
+
+/* collisions_static.cu:345-442 addTECollsion: edge (e0, e1) of one triangle against the
+ * plane and barycentric interior of `t`.  Returns 1 and sets *loop / *coef when a row is
+ * emitted.  Both the parametric t and the barycentric test let NaN through, as in the
+ * reference (every comparison with NaN is false). */
+static inline int FN(te_intersect)(TRIT t, V3T e0, V3T e1, int cid, int64_t *loop, REAL *coef) {
+    V3T edge1 = FN(vsub)(t.v[1], t.v[0]), edge2 = FN(vsub)(t.v[2], t.v[0]);
+    V3T nrm = FN(vnormalize)(FN(vcross)(edge1, edge2));
+    int s0 = FN(has_vertex)(t, e0), s1 = FN(has_vertex)(t, e1);
+    if (s0 || s1) {                         /* :385-405 loop node: only cid < 3 with e0 shared */
+        if (cid > 2 || !s0) return 0;
+        *loop = 0;
+        *coef = (REAL)(float)0.0;
+        return 1;
+    }
+    V3T dir = FN(vsub)(e1, e0);
+    REAL en = FN(vdot)(dir, nrm);
+    if (en == (REAL)0) return 0;            /* :410-414 parallel */
+    REAL d = FN(vdot)(nrm, t.v[0]);
+    REAL tp = (d - FN(vdot)(nrm, e0)) / en;
+    if (tp < (REAL)0 || tp > (REAL)1) return 0;
+    V3T ip = FN(vadd)(e0, FN(vmul)(dir, tp));
+    V3T ep = FN(vsub)(ip, t.v[0]);
+    REAL d00 = FN(vdot)(edge1, edge1), d01 = FN(vdot)(edge1, edge2), d11 = FN(vdot)(edge2, edge2);
+    REAL d20 = FN(vdot)(ep, edge1), d21 = FN(vdot)(ep, edge2);
+    REAL denom = d00 * d11 - d01 * d01;
+    if (denom < (REAL)1e-15) return 0;
+    REAL v = (d11 * d20 - d01 * d21) / denom;
+    REAL w = (d00 * d21 - d01 * d20) / denom;
+    REAL u = (REAL)1.0 - v - w;
+    if (u < (REAL)0 || v < (REAL)0 || w < (REAL)0) return 0;
+    *loop = -1;
+    *coef = tp;
+    return 1;
+}
+
+/* Triangle-edge queries over candidate pairs (x, y):
+ *   mode 0 find_triangle_edge_collisions  (collisions_static.cu:445-500, :612-779):
+ *          pairs that pass SAT with no shared vertex; rows (x, y, cid, loop) + coefficient;
+ *   mode 1 find_triangle_edge_candidates  (collisions_static.cu:520-604, :782-943):
+ *          pairs that pass SAT with at most one shared vertex; rows (x, y, cid, loop);
+ *   mode 2 find_triangle_edge_candidates2 (icontour.cu:11-108, :110-282): the same pairs,
+ *          one row (x, y) each with bit masks (mx, my).
+ * cid 0..2 are the edges (v0v1, v1v2, v2v0) of x against y, 3..5 those of y against x.
+ * The candidate filter's box test compares the second triangle with the box of its own
+ * edge (collisions_static.cu:594-596), which always holds for finite input; it is kept as
+ * written. */
+int64_t FN(orc_triangle_edge)(const REAL *tris, int64_t n, const int64_t *pairs, int64_t k, int mode,
+                              int64_t *rows, REAL *coefs, int32_t *masks, int64_t cap) {
+    (void)n;
+    int64_t R = 0;
+    const int allow = mode == 0 ? 0 : 1;
+    for (int64_t c = 0; c < k; ++c) {
+        int64_t x = pairs[2 * c], y = pairs[2 * c + 1];
+        TRIT a = FN(load_tri)(tris, x), b = FN(load_tri)(tris, y);
+        if (!(FN(sat)(a, b, (REAL)0) && !(FN(count_shared)(a.v, b.v) > allow))) continue;
+        int32_t mx = 0, my = 0;
+        for (int cid = 0; cid < 6; ++cid) {
+            TRIT t1 = cid < 3 ? a : b, t2 = cid < 3 ? b : a;
+            int e = cid % 3;
+            V3T e0 = t1.v[e], e1 = t1.v[(e + 1) % 3];
+            int64_t loop = -1;
+            REAL coef = (REAL)-1;
+            int emit = 0;
+            if (mode == 0) {
+                emit = FN(te_intersect)(t2, e0, e1, cid, &loop, &coef);
+            } else {
+                int s0 = FN(has_vertex)(t2, e0), s1 = FN(has_vertex)(t2, e1);
+                if (s0) {
+                    if (cid < 3) { emit = 1; loop = 0; mx |= 1 << cid; my |= 1 << cid; }
+                } else if (!s1) {
+                    BOXT b0 = FN(tri_box)(t2), b1 = FN(tri_ebox)(t2, e, (e + 1) % 3);
+                    if (FN(check_overlap)(b0, b1, (REAL)0)) { emit = 1; mx |= 1 << cid; }
+                }
+            }
+            if (emit && mode != 2) {
+                if (R < cap) {
+                    rows[4 * R] = x; rows[4 * R + 1] = y; rows[4 * R + 2] = cid; rows[4 * R + 3] = loop;
+                    if (coefs) coefs[R] = coef;
+                }
+                R++;
+            }
+        }
+        if (mode == 2) {
+            if (R < cap) { rows[2 * R] = x; rows[2 * R + 1] = y; masks[2 * R] = mx; masks[2 * R + 1] = my; }
             R++;
         }
     }

@@ -68,6 +68,8 @@ def _declare(L):
         getattr(L, "orc_proximity" + sfx).restype = C.c_int
         getattr(L, "orc_static_collisions" + sfx).argtypes = [_P, _I64, _P, _I64, real, _P, _I64]
         getattr(L, "orc_static_collisions" + sfx).restype = _I64
+        getattr(L, "orc_triangle_edge" + sfx).argtypes = [_P, _I64, _P, _I64, C.c_int, _P, _P, _P, _I64]
+        getattr(L, "orc_triangle_edge" + sfx).restype = _I64
         getattr(L, "orc_mincol" + sfx).argtypes = [_P, _P, _I64, _P, _I64, _P, _I64]
         getattr(L, "orc_mincol" + sfx).restype = _I64
         getattr(L, "orc_cubic_coeffs" + sfx).argtypes = [_P, _P]
@@ -224,6 +226,33 @@ def static_collisions(tris, pairs, threshold):
     return out[:total]
 
 
+TE_MODES = {"collisions": 0, "candidates": 1, "candidates2": 2}
+
+
+def triangle_edge(tris, pairs, mode):
+    """Triangle-edge rows over candidate pairs (collisions_static.cu:345-943, icontour.cu).
+    mode "collisions" -> (rows [R,4], coefs [R]); "candidates" -> rows [R,4];
+    "candidates2" -> (pairs [R,2], masks [R,2] int32)."""
+    tris = _c(tris)
+    pairs = _c(np.asarray(pairs).reshape(-1, 2), np.int64)
+    n = tris.shape[0]
+    sfx, _ = _sfx(tris)
+    m = TE_MODES[mode]
+    f = getattr(lib(), "orc_triangle_edge" + sfx)
+    total = f(_ptr(tris), n, _ptr(pairs), pairs.shape[0], m, None, None, None, 0)
+    width = 2 if m == 2 else 4
+    rows = np.empty((max(total, 1), width), np.int64)
+    coefs = np.empty(max(total, 1), tris.dtype)
+    masks = np.empty((max(total, 1), 2), np.int32)
+    f(_ptr(tris), n, _ptr(pairs), pairs.shape[0], m, _ptr(rows), _ptr(coefs) if m == 0 else None,
+      _ptr(masks) if m == 2 else None, total)
+    if m == 0:
+        return rows[:total], coefs[:total]
+    if m == 1:
+        return rows[:total]
+    return rows[:total], masks[:total]
+
+
 def mincol(tris, tris_next, pairs):
     tris = _c(tris)
     tris_next = _c(tris_next, tris.dtype)
@@ -323,3 +352,10 @@ def find_collisions(bboxes, tree, tris, threshold=0.0):
     thr = float(np.float32(threshold))
     pairs, _ = traverse(bboxes, tree, thr)
     return static_collisions(tris, pairs, thr)
+
+
+def find_triangle_edge(bboxes, tree, tris, mode):
+    """The three triangle-edge queries end to end: traversal and SAT at threshold 0 (the
+    bindings' threshold argument is not used, collisions_static.cu:681, :733)."""
+    pairs, _ = traverse(bboxes, tree, 0.0)
+    return triangle_edge(tris, pairs, mode)

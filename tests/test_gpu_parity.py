@@ -115,6 +115,64 @@ def test_compute_impulses(ccc, oracle, name):
     _check_close(_np(g[1])[0], dx, "dx")
 
 
+PACKED_MESHES = [
+    lambda: cases.folded(8, seed=1),
+    lambda: cases.random_soup(300, seed=8),
+    lambda: cases.random_soup(1, seed=9),
+    lambda: cases.folded(14, seed=3),
+    lambda: cases.shared_vertex_soup(60),
+    lambda: cases.random_soup(2, seed=10),
+]
+
+
+def test_packed_meshes_match_single_mesh_oracle(ccc, oracle):
+    """compute_impulses_packed over meshes of different sizes (1..~400 triangles): each
+    mesh's cnt equals its own single-mesh oracle bit for bit, dv/dx to 1e-9, and the batch
+    counts are the sums of the per-mesh counts (no pair crosses a mesh)."""
+    meshes = [f() for f in PACKED_MESHES]
+    offs = np.cumsum([0] + [len(m[0]) for m in meshes]).tolist()
+    t0 = np.concatenate([m[0] for m in meshes])
+    t1 = np.concatenate([m[1] for m in meshes])
+    ms = np.concatenate([m[2] for m in meshes])
+    dv, dx, cnt = ccc.compute_impulses_packed(_t(t0), _t(t1), _t(ms), offs, 512)
+    K, M, V, R = ccc.last_counts()[0]
+    tot = np.zeros(4, np.int64)
+    for i, (a0, a1, am) in enumerate(meshes):
+        lo, hi = offs[i], offs[i + 1]
+        rdv, rdx, rcnt, _, _, rc = oracle.compute_impulses(*oracle.bvh(a0, a1)[:2], a0, a1, am)
+        tot += [rc["candidates"], rc["subprims"], rc["roots"], rc["collisions"]]
+        np.testing.assert_array_equal(_np(cnt)[lo:hi], rcnt)
+        _check_close(_np(dv)[lo:hi], rdv, f"dv mesh {i}")
+        _check_close(_np(dx)[lo:hi], rdx, f"dx mesh {i}")
+    assert (K, M, V, R) == tuple(tot)
+    assert R > 0
+
+
+def test_packed_alone_equals_in_the_mix(ccc):
+    """Each mesh alone (plain compute_impulses on its own BVH) and inside the packed batch:
+    same counts of impulses per corner, dv/dx within 1e-9."""
+    meshes = [f() for f in PACKED_MESHES]
+    offs = np.cumsum([0] + [len(m[0]) for m in meshes]).tolist()
+    cat = [np.concatenate([m[k] for m in meshes]) for k in range(3)]
+    dv, dx, cnt = ccc.compute_impulses_packed(_t(cat[0]), _t(cat[1]), _t(cat[2]), offs, 512)
+    for i, (a0, a1, am) in enumerate(meshes):
+        bb, tree = ccc.bvh_motion(_t(a0[None]), _t(a1[None]))
+        sdv, sdx, scnt = ccc.compute_impulses(bb, tree, _t(a0[None]), _t(a1[None]), _t(am[None]), 512)
+        lo, hi = offs[i], offs[i + 1]
+        np.testing.assert_array_equal(_np(cnt)[lo:hi], _np(scnt)[0])
+        _check_close(_np(dv)[lo:hi], _np(sdv)[0], f"dv mesh {i}")
+        _check_close(_np(dx)[lo:hi], _np(sdx)[0], f"dx mesh {i}")
+
+
+def test_packed_rejects_bad_offsets(ccc):
+    t = _t(np.zeros((4, 3, 3)))
+    m = _t(np.ones((4, 3)))
+    with pytest.raises(RuntimeError, match="offsets"):
+        ccc.compute_impulses_packed(t, t, m, [0, 2, 3])
+    with pytest.raises(RuntimeError, match="strictly increasing"):
+        ccc.compute_impulses_packed(t, t, m, [0, 2, 2, 4])
+
+
 def test_impulses_from_collisions_with_padding(ccc, oracle):
     t0, t1, m = cases.folded(20)
     rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
@@ -222,6 +280,82 @@ def test_static_sat(ccc, oracle, thr):
     ref = oracle.find_collisions(rb, rt, t0, thr)
     g = ccc.find_collisions(_t(rb[None]), _t(rt[None]), _t(t0[None]), thr, 64)
     np.testing.assert_array_equal(sorted_rows(valid_rows(_np(g)[0])), sorted_rows(ref))
+
+
+TE_SCENES = {
+    "random300": lambda dt: cases.random_soup(300, seed=7, dtype=dt)[0],
+    "shared200": lambda dt: cases.shared_vertex_soup(200, dtype=dt)[0],
+    "fold": lambda dt: cases.proximity_fold(16, dtype=dt),
+}
+
+
+def _te_rows(a, width):
+    a = np.asarray(a).reshape(-1, width)
+    return a[a[:, 0] >= 0]
+
+
+@pytest.mark.parametrize("name", list(TE_SCENES))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_triangle_edge_queries(ccc, oracle, name, dtype):
+    """All three triangle-edge queries: row multisets bit-exact, edge coefficients (per row)
+    bit-exact, pair/mask rows of candidates2 bit-exact; threshold is ignored as in the
+    reference."""
+    t0 = TE_SCENES[name](dtype)
+    n = t0.shape[0]
+    rb, rt, _, _ = oracle.bvh(t0)
+    # capacity for every candidate pair and every row (the shared-vertex soup needs ~190 rows
+    # per triangle)
+    pairs, _ = oracle.traverse(rb, rt, 0.0)
+    mc = max(64, 6 * int(np.ceil(pairs.shape[0] / n)) + 1)
+    g, gc = ccc.find_triangle_edge_collisions(_t(rb[None]), _t(rt[None]), _t(t0[None]), 0.5, mc)
+    rows, coefs = oracle.find_triangle_edge(rb, rt, t0, "collisions")
+    gr = _np(g)[0]
+    keep = gr[:, 0] >= 0
+    got = sorted_rows(np.concatenate([gr[keep], _np(gc)[0][keep].view(np.int64 if dtype == np.float64 else np.int32)
+                                      .astype(np.int64)], axis=1))
+    want = sorted_rows(np.concatenate([rows, coefs.reshape(-1, 1).view(np.int64 if dtype == np.float64 else np.int32)
+                                       .astype(np.int64)], axis=1))
+    np.testing.assert_array_equal(got, want)
+    assert (_np(gc)[0][~keep] == -1).all()
+
+    g = ccc.find_triangle_edge_candidates(_t(rb[None]), _t(rt[None]), _t(t0[None]), 0.0, mc)
+    ref = oracle.find_triangle_edge(rb, rt, t0, "candidates")
+    np.testing.assert_array_equal(sorted_rows(_te_rows(_np(g)[0], 4)), sorted_rows(ref))
+
+    gp, gm = ccc.find_triangle_edge_candidates2(_t(rb[None]), _t(rt[None]), _t(t0[None]), 0.0, mc)
+    rp, rm = oracle.find_triangle_edge(rb, rt, t0, "candidates2")
+    gp, gm = _np(gp)[0], _np(gm)[0]
+    keep = gp[:, 0] >= 0
+    np.testing.assert_array_equal(sorted_rows(np.concatenate([gp[keep], gm[keep]], axis=1)),
+                                  sorted_rows(np.concatenate([rp, rm.astype(np.int64)], axis=1)))
+    assert (gm[~keep] == 0).all() and keep.sum() == rp.shape[0]
+    assert ref.shape[0] > 0 and n > 0
+
+
+def test_triangle_edge_batch_and_truncation(ccc, oracle):
+    a = cases.random_soup(300, seed=7)[0]
+    b = cases.shared_vertex_soup(300)[0]
+    t0 = np.stack([a, b])
+    bbs, trees = [], []
+    for i in range(2):
+        rb, rt, _, _ = oracle.bvh(t0[i])
+        bbs.append(rb)
+        trees.append(rt)
+    refs = [oracle.find_triangle_edge(bbs[i], trees[i], t0[i], "candidates") for i in range(2)]
+    kmax = max(oracle.traverse(bbs[i], trees[i], 0.0)[0].shape[0] for i in range(2))
+    mc = max(6 * int(np.ceil(kmax / 300)) + 1, int(np.ceil(max(r.shape[0] for r in refs) / 300)) + 1)
+    g = ccc.find_triangle_edge_candidates(_t(np.stack(bbs)), _t(np.stack(trees)), _t(t0), 0.0, mc)
+    for i in range(2):
+        np.testing.assert_array_equal(sorted_rows(_te_rows(_np(g)[i], 4)), sorted_rows(refs[i]))
+    # 1 row per triangle of capacity: the shared-vertex soup has more candidate rows
+    ref = refs[1]
+    assert ref.shape[0] > 300
+    with pytest.warns(UserWarning):
+        g = ccc.find_triangle_edge_candidates(_t(bbs[1][None]), _t(trees[1][None]), _t(t0[1][None]), 0.0, 1)
+    got = _te_rows(_np(g)[0], 4)
+    assert got.shape[0] == 300
+    want = {tuple(r) for r in ref.tolist()}
+    assert all(tuple(r) in want for r in got.tolist())
 
 
 def test_mincol(ccc, oracle):

@@ -96,13 +96,15 @@ def test_inputs_must_be_device_tensors():
         ccc.compute_impulses(bb, tree, t, t, torch.zeros((1, 4, 3), dtype=torch.float64))
 
 
-def test_triangle_edge_queries_report_not_implemented():
+def test_triangle_edge_queries_validate_inputs():
     import torch
     import cccollisions_amd as ccc
     t = torch.zeros((1, 4, 3, 3), dtype=torch.float64)
+    bb = torch.zeros((1, 7, 6), dtype=torch.float64)
+    tree = torch.zeros((1, 7, 4), dtype=torch.int64)
     for name in ("find_triangle_edge_collisions", "find_triangle_edge_candidates", "find_triangle_edge_candidates2"):
-        with pytest.raises(NotImplementedError):
-            getattr(ccc, name)(t, t, t)
+        with pytest.raises(RuntimeError, match="bboxes must be a CUDA tensor"):
+            getattr(ccc, name)(bb, tree, t)
 
 
 def test_product_never_imports_the_oracle():
