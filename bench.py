@@ -44,6 +44,14 @@ def parse():
                     help="also time the packed multi-mesh path against the plain per-mesh path on this many "
                          "folded cloths of 2k-200k triangles (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=1)
+    ap.add_argument("--serving-sims", type=int, default=0,
+                    help="also run the serving loop over this many resident folded cloths (lognormal sizes, "
+                         "median 20k triangles; 0: skip)")
+    ap.add_argument("--serving-max-tris", type=int, default=200000)
+    ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--serving-load", type=float, default=0.7,
+                    help="offered load of the open-loop run as a fraction of the saturation rate")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy dispatch of "
                          "known size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -65,10 +73,10 @@ class Step:
         self.nv = scene.x0.shape[0]
         self.counts = []
 
-    def run(self):
+    def run(self, iters=None):
         torch, ccc = self.torch, self.ccc
         x1 = self.x1
-        for _ in range(self.iters):
+        for _ in range(self.iters if iters is None else iters):
             t1 = x1[self.faces].unsqueeze(0).contiguous()
             bb, tree = ccc.bvh_motion(self.t0, t1)
             dv, dx, cnt = ccc.compute_impulses(bb, tree, self.t0, t1, self.masses, self.maxc, 16)
@@ -105,10 +113,10 @@ class BatchStep:
         self.counts = []
         self.first_pass = None      # positions after the very first impulse pass
 
-    def run(self):
+    def run(self, iters=None):
         torch, ccc = self.torch, self.ccc
         x1 = self.x1
-        for _ in range(self.iters):
+        for _ in range(self.iters if iters is None else iters):
             if self.first_pass is None and self.counts:
                 self.first_pass = x1.clone()
             t1 = x1[self.faces].contiguous()
@@ -136,9 +144,9 @@ class BatchStep:
         return x1
 
 
-def time_steps(torch, step, steps, warmup):
+def time_steps(torch, step, steps, warmup, warmup_iters=None):
     for _ in range(warmup):
-        step.run()
+        step.run(warmup_iters)
     torch.cuda.synchronize()
     lat = []
     t_start = time.perf_counter()
@@ -159,8 +167,10 @@ def batched_section(ccc, torch, scenes, args):
     tris = int(sum(sc.num_triangles for sc in sims))
     packed = BatchStep(ccc, torch, sims, args.iters, args.max_candidates, True)
     plain = BatchStep(ccc, torch, sims, args.iters, args.max_candidates, False)
-    tp, latp, _ = time_steps(torch, packed, args.batched_steps, 1)
-    tq, latq, _ = time_steps(torch, plain, args.batched_steps, 1)
+    # warm-up: two impulse passes each (allocator, code paths); the first one is also the
+    # first pass of the comparison below
+    tp, latp, _ = time_steps(torch, packed, args.batched_steps, 1, 2)
+    tq, latq, _ = time_steps(torch, plain, args.batched_steps, 1, 2)
     # after one pass from the same input the two paths differ only by the order of the fp64
     # atomic sums; later passes amplify such differences through borderline collisions
     moved = float((plain.first_pass - plain.x1).abs().max())
@@ -178,6 +188,46 @@ def batched_section(ccc, torch, scenes, args):
         "first_pass_counts_equal": list(map(int, packed.counts[0])) == list(map(int, plain.counts[0])),
         "first_pass_max_position_difference": diff, "first_pass_max_correction": moved,
     }
+
+
+def serving_section(ccc, torch, scenes, args):
+    """Resident folded cloths with lognormal sizes (median 20k triangles), Poisson request
+    arrivals; each request is one CCD + impulse step of one simulation.  First the
+    saturation throughput (every request queued at t = 0), then an open-loop run at
+    args.serving_load of that rate: completed steps/s and p50 / p99 latency."""
+    from cccollisions_amd import serving
+    rng = np.random.default_rng(5)
+    sizes = np.minimum(np.exp(rng.normal(np.log(20000), 0.8, args.serving_sims)), args.serving_max_tris)
+    cl = [scenes.folded_cloth(scenes.grid_size_for_triangles(int(s)), seed=500 + i) for i, s in enumerate(sizes)]
+    sims = [serving.Simulation(torch, sc.faces, sc.x0, sc.x1, sc.corner_masses()) for sc in cl]
+    tris = [s.num_triangles for s in sims]
+
+    def run(arrivals, sim_ids):
+        server = serving.CollisionServer(ccc, torch, sims, slots=args.serving_slots, iters=args.iters,
+                                         max_candidates=args.max_candidates)
+        t0 = time.perf_counter()
+        reqs = []
+        for a, s in zip(arrivals, sim_ids):
+            while time.perf_counter() - t0 < a:
+                time.sleep(min(0.002, a - (time.perf_counter() - t0)))
+            reqs.append(server.submit(int(s), arrival=server.now()))
+        server.close()
+        span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+        lat = np.array([r.latency for r in reqs])
+        return {"requests": len(reqs), "steps_per_s": len(reqs) / span,
+                "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                "mean_batch": float(np.mean([r.batch_size for r in reqs]))}
+
+    # warm-up (allocator, code paths) with one request per simulation
+    run(np.zeros(len(sims)), np.arange(len(sims)))
+    n_sat = args.serving_requests
+    sat = run(np.zeros(n_sat), rng.integers(0, len(sims), n_sat))
+    rate = args.serving_load * sat["steps_per_s"]
+    gaps = rng.exponential(1.0 / rate, n_sat)
+    open_loop = run(np.cumsum(gaps) - gaps[0], rng.integers(0, len(sims), n_sat))
+    return {"sims": len(sims), "triangles_median": int(np.median(tris)), "triangles_max": int(max(tris)),
+            "triangles_total": int(sum(tris)), "slots": args.serving_slots, "impulse_iterations": args.iters,
+            "saturation": sat, "offered_steps_per_s": rate, "open_loop": open_loop}
 
 
 def roofline(kt, counts, steps, iters):
@@ -292,6 +342,8 @@ def main():
     }
     if args.batched_sims:
         out["batched"] = batched_section(ccc, torch, scenes, args)
+    if args.serving_sims:
+        out["serving"] = serving_section(ccc, torch, scenes, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_stride, args.iters)
     print(json.dumps(out), flush=True)
