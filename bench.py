@@ -44,6 +44,9 @@ def parse():
                     help="also time the packed multi-mesh path against the plain per-mesh path on this many "
                          "folded cloths of 2k-200k triangles (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=1)
+    ap.add_argument("--garment", action="store_true",
+                    help="also time the garment-on-body scene (static 24k-triangle capsule body + 100k-triangle "
+                         "sleeve moving inward)")
     ap.add_argument("--serving-sims", type=int, default=0,
                     help="also run the serving loop over this many resident folded cloths (lognormal sizes, "
                          "median 20k triangles; 0: skip)")
@@ -158,6 +161,20 @@ def time_steps(torch, step, steps, warmup, warmup_iters=None):
     return time.perf_counter() - t_start, lat, out
 
 
+def garment_section(ccc, torch, scenes, args):
+    """Garment vs body: 24k-triangle static capsule body + 100k-triangle sleeve around the
+    right arm at radius + 2e-3 with inward speeds |N(0, 0.3)|, one soup, fp64; a step is
+    `iters` impulse passes as for the cloth."""
+    sc = scenes.garment_on_body()
+    step = Step(ccc, torch, sc, args.iters, args.max_candidates)
+    el, lat, _ = time_steps(torch, step, args.steps, 1)
+    timed = step.counts[-args.steps * args.iters:]
+    return {"triangles": sc.num_triangles, "steps": args.steps, "steps_per_s": args.steps / el,
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "first_pass_counts": list(map(int, step.counts[0])),
+            "mean_collisions_per_iter": float(np.mean([c[3] for c in timed]))}
+
+
 def batched_section(ccc, torch, scenes, args):
     """Folded cloths of 2k-200k triangles (log-uniform sizes, seeded): the packed path
     against the plain per-mesh path on the same simulation.  Reports how far apart the
@@ -230,6 +247,18 @@ def serving_section(ccc, torch, scenes, args):
             "saturation": sat, "offered_steps_per_s": rate, "open_loop": open_loop}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (profiles/traffic_<kernel>_folded_cloth_130k.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_folded_cloth_130k.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    except (OSError, IndexError, KeyError, ValueError):
+        return None, None
+
+
 def roofline(kt, counts, steps, iters):
     """Roofline of the dominant kernel over the timed region."""
     name, (ms, launches) = max(kt.items(), key=lambda kv: kv[1][0])
@@ -240,10 +269,12 @@ def roofline(kt, counts, steps, iters):
     if name.startswith("ccd"):
         work = FLOPS_PER_SUBPRIM * m_total / max(launches, 1)
         achieved = work / avg_s / 1e12
+        traffic, src = measured_traffic("k_ccd")
         return name, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                       "kernel": name, "avg_launch_ms": avg_s * 1e3,
-                      "work_per_launch": f"{FLOPS_PER_SUBPRIM} fp64 flop x {m_total / max(launches, 1):.4g} VF/EE pairs"}
+                      "work_per_launch": f"{FLOPS_PER_SUBPRIM} fp64 flop x {m_total / max(launches, 1):.4g} VF/EE pairs",
+                      "traffic_source": src}
     # memory-bound kernels: unique bytes per launch
     per = {"traverse": 64.0, "subprim": 8.0}
     b = per.get(name, 0.0) * (k_total / max(launches, 1))
@@ -340,6 +371,8 @@ def main():
         "roofline": roof,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items())},
     }
+    if args.garment:
+        out["garment"] = garment_section(ccc, torch, scenes, args)
     if args.batched_sims:
         out["batched"] = batched_section(ccc, torch, scenes, args)
     if args.serving_sims:

@@ -183,13 +183,14 @@ __device__ __forceinline__ int lcp_delta(const uint32_t *codes, const int32_t *i
 // direction, the far end j by exponential + binary search and the split by binary search.
 // Writes left/right/self of row i and the parent field of both children's rows.
 __global__ void k_karras(const uint32_t *__restrict__ codes_all, const int32_t *__restrict__ ids_all, int64_t n,
-                         int64_t *__restrict__ tree_all) {
+                         int64_t *__restrict__ tree_all, int2 *__restrict__ range_all) {
     const int b = blockIdx.y;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const uint32_t *codes = codes_all + (int64_t)b * n;
     const int32_t *ids = ids_all + (int64_t)b * n;
     int64_t *tree = tree_all + (int64_t)b * (2 * n - 1) * 4;
+    int2 *range = range_all + (int64_t)b * (n - 1);
     const int d = (lcp_delta(codes, ids, n, i, i + 1) - lcp_delta(codes, ids, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = lcp_delta(codes, ids, n, i, i - d);
     int64_t lmax = 2;
@@ -215,66 +216,103 @@ __global__ void k_karras(const uint32_t *__restrict__ codes_all, const int32_t *
     if (i == 0) tree[0] = -1;
     tree[4 * left + 0] = i;
     tree[4 * right + 0] = i;
+    range[i] = make_int2((int)lo, (int)hi);
 }
 
-__device__ __forceinline__ double ld_coherent(const double *p) {
-    unsigned long long v = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)v);
-}
-__device__ __forceinline__ float ld_coherent(const float *p) {
-    unsigned v = __hip_atomic_load((const unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __uint_as_float(v);
-}
-__device__ __forceinline__ int64_t ld_coherent_i64(const int64_t *p) {
-    return (int64_t)__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Node boxes (build_bvh.cu:145-209 computes them by an atomic bottom-up climb).  Every
+// internal node of the Karras tree covers a contiguous range [lo, hi] of sorted leaf
+// positions, so its box is the union of the leaf boxes of that range.  Min / max are exact
+// and order-independent, so any grouping gives the climb's result; here the union is read
+// off a hierarchy of chunk boxes (64, 64^2, ... leaves), with no atomics or fences.
+constexpr int kChunkBits = 6;
+constexpr int kChunk = 1 << kChunkBits;
+constexpr int kChunkLevels = 5;      // 64^5 = 2^30 > 2^28 leaves
 
-// Leaf rows and bottom-up refit (build_bvh.cu:145-209): leaf at sorted position p gets the
-// box of triangle ids[p]; the second thread to reach an internal node writes the union of
-// its children and climbs on.  A climb is bounded by n steps.
+// Leaf rows (build_bvh.cu:309-378 layout): the box of triangle ids[p] and (-1, -1, id).
 template <typename T>
-__global__ void k_refit(const T *__restrict__ boxes_all, const int32_t *__restrict__ ids_all, int64_t n,
-                        int *__restrict__ counters_all, T *__restrict__ bboxes_all, int64_t *__restrict__ tree_all,
-                        unsigned *__restrict__ err) {
+__global__ void k_leaf_rows(const T *__restrict__ boxes_all, const int32_t *__restrict__ ids_all, int64_t n,
+                            T *__restrict__ bboxes_all, int64_t *__restrict__ tree_all) {
     const int b = blockIdx.y;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t rows = 2 * n - 1;
     const T *boxes = boxes_all + (int64_t)b * n * 6;
-    const int32_t *ids = ids_all + (int64_t)b * n;
-    int *counters = counters_all + (int64_t)b * (n > 1 ? n - 1 : 1);
     T *bb = bboxes_all + (int64_t)b * rows * 6;
     int64_t *tree = tree_all + (int64_t)b * rows * 4;
     const int64_t row = n - 1 + p;
-    const int32_t id = ids[p];
-    const T *src = boxes + 6 * (int64_t)id;
+    const int32_t id = ids_all[(int64_t)b * n + p];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) bb[6 * row + k] = src[k];
+    for (int k = 0; k < 6; ++k) bb[6 * row + k] = boxes[6 * (int64_t)id + k];
+    if (n == 1) tree[4 * row + 0] = -1;
     tree[4 * row + 1] = -1;
     tree[4 * row + 2] = -1;
     tree[4 * row + 3] = id;
-    if (n == 1) {
-        tree[4 * row + 0] = -1;
-        return;
+}
+
+template <typename T> __device__ __forceinline__ void box_union_into(T *acc, const T *src) {
+    acc[0] = tmin(acc[0], src[0]);
+    acc[1] = tmin(acc[1], src[1]);
+    acc[2] = tmin(acc[2], src[2]);
+    acc[3] = tmax(acc[3], src[3]);
+    acc[4] = tmax(acc[4], src[4]);
+    acc[5] = tmax(acc[5], src[5]);
+}
+
+// chunk level L box c = union of the (at most 64) level L-1 boxes 64c .. 64c+63; level 0 is
+// the leaf rows of bboxes.
+template <typename T>
+__global__ void k_chunk_boxes(const T *__restrict__ src_all, int64_t src_stride, int64_t src_count, T *__restrict__ dst_all,
+                              int64_t dst_stride, int64_t dst_count) {
+    const int b = blockIdx.y;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= dst_count) return;
+    const T *src = src_all + (int64_t)b * src_stride;
+    T acc[6];
+    const int64_t first = c << kChunkBits;
+    const int64_t last = first + kChunk < src_count ? first + kChunk : src_count;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] = src[6 * first + k];
+    for (int64_t j = first + 1; j < last; ++j) box_union_into(acc, src + 6 * j);
+    T *dst = dst_all + (int64_t)b * dst_stride + 6 * c;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = acc[k];
+}
+
+struct ChunkLevels {
+    const void *box[kChunkLevels + 1];   // [0]: leaf boxes (bboxes leaf rows), [L]: level L chunks
+    int64_t stride[kChunkLevels + 1];    // elements between batch elements
+    int levels;                          // chunk levels in use
+};
+
+template <typename T>
+__global__ void k_node_boxes(const int2 *__restrict__ range_all, int64_t n, ChunkLevels lv, T *__restrict__ bboxes_all) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 r = range_all[(int64_t)b * (n - 1) + i];
+    int64_t p = r.x;
+    const int64_t end = (int64_t)r.y + 1;
+    T acc[6];
+    bool first = true;
+    while (p < end) {
+        // the largest aligned chunk that starts at p and ends inside the range
+        int L = 0;
+        while (L < lv.levels && (p & ((int64_t)kChunk << (kChunkBits * L)) - 1) == 0 &&
+               p + ((int64_t)kChunk << (kChunkBits * L)) <= end)
+            ++L;
+        const T *src = (const T *)lv.box[L] + (int64_t)b * lv.stride[L] + 6 * (p >> (kChunkBits * L));
+        if (first) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] = src[k];
+            first = false;
+        } else {
+            box_union_into(acc, src);
+        }
+        p += (int64_t)1 << (kChunkBits * L);
     }
-    int64_t cur = tree[4 * row + 0];
-    for (int64_t step = 0; cur >= 0; ++step) {
-        if (step > n) { atomicOr(err, DEV_BAD_TREE); break; }
-        __threadfence();
-        const int old = __hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) break;
-        __threadfence();
-        const int64_t l = tree[4 * cur + 1], r = tree[4 * cur + 2];
-        Box<T> bl, br;
-        bl.mn = mk(ld_coherent(bb + 6 * l + 0), ld_coherent(bb + 6 * l + 1), ld_coherent(bb + 6 * l + 2));
-        bl.mx = mk(ld_coherent(bb + 6 * l + 3), ld_coherent(bb + 6 * l + 4), ld_coherent(bb + 6 * l + 5));
-        br.mn = mk(ld_coherent(bb + 6 * r + 0), ld_coherent(bb + 6 * r + 1), ld_coherent(bb + 6 * r + 2));
-        br.mx = mk(ld_coherent(bb + 6 * r + 3), ld_coherent(bb + 6 * r + 4), ld_coherent(bb + 6 * r + 5));
-        Box<T> u = unite(bl, br);
-        T *o = bb + 6 * cur;
-        o[0] = u.mn.x; o[1] = u.mn.y; o[2] = u.mn.z; o[3] = u.mx.x; o[4] = u.mx.y; o[5] = u.mx.z;
-        cur = tree[4 * cur + 0];
-    }
+    T *o = bboxes_all + (int64_t)b * (2 * n - 1) * 6 + 6 * i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = acc[k];
 }
 
 template <typename T>
@@ -288,12 +326,18 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     uint32_t *ka = ws.get_as<uint32_t>(total), *kb = ws.get_as<uint32_t>(total);
     int32_t *va = ws.get_as<int32_t>(total), *vb = ws.get_as<int32_t>(total);
     uint32_t *hist = ws.get_as<uint32_t>((size_t)B * 256 * nb);
-    int *counters = ws.get_as<int>((size_t)B * (n > 1 ? n - 1 : 1));
-    unsigned *err = ws.get_as<unsigned>(1);
+    int2 *range = ws.get_as<int2>((size_t)B * (n > 1 ? n - 1 : 1));
+    ChunkLevels lv{};
+    lv.box[0] = bboxes + (size_t)6 * (n - 1);
+    lv.stride[0] = (2 * n - 1) * 6;
+    int64_t count = n;
+    while (lv.levels < kChunkLevels && count > kChunk) {
+        count = (count + kChunk - 1) / kChunk;
+        ++lv.levels;
+        lv.box[lv.levels] = ws.get_as<T>((size_t)B * count * 6);
+        lv.stride[lv.levels] = count * 6;
+    }
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
-    ccc_status st;
-    if ((st = fill_u32(counters, B * (n > 1 ? n - 1 : 1), 0u, s)) != CCC_OK) return st;
-    if ((st = fill_u32(err, 1, 0u, s)) != CCC_OK) return st;
 
     tm.begin("bvh_leaf_boxes");
     if (t1) hipLaunchKernelGGL((k_leaf_boxes<T, true>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
@@ -313,19 +357,24 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     }
     tm.end();
     if (n > 1) {
-        CCC_TIMED(tm, "bvh_karras", hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree));
+        CCC_TIMED(tm, "bvh_karras", hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree,
+                                                       range));
         CCC_LAUNCH_CHECK("k_karras");
     }
-    CCC_TIMED(tm, "bvh_refit",
-              hipLaunchKernelGGL((k_refit<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, boxes, va, n, counters, bboxes, tree, err));
-    CCC_LAUNCH_CHECK("k_refit");
-    unsigned herr = 0;
-    CCC_HIP_TRY(hipMemcpyAsync(&herr, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    CCC_HIP_TRY(hipStreamSynchronize(s));
-    if (herr) {
-        set_error("BVH refit did not terminate (corrupt tree)");
-        return CCC_ERR_BAD_TREE;
+    tm.begin("bvh_refit");
+    hipLaunchKernelGGL((k_leaf_rows<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, boxes, va, n, bboxes, tree);
+    {
+        int64_t src_count = n;
+        for (int L = 1; L <= lv.levels; ++L) {
+            const int64_t dst_count = (src_count + kChunk - 1) / kChunk;
+            hipLaunchKernelGGL((k_chunk_boxes<T>), dim3(grid_for(dst_count, 256), B), dim3(256), 0, s, (const T *)lv.box[L - 1],
+                               lv.stride[L - 1], src_count, (T *)lv.box[L], lv.stride[L], dst_count);
+            src_count = dst_count;
+        }
     }
+    if (n > 1) hipLaunchKernelGGL((k_node_boxes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, range, n, lv, bboxes);
+    tm.end();
+    CCC_LAUNCH_CHECK("node boxes");
     return CCC_OK;
 }
 

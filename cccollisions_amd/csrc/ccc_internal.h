@@ -125,7 +125,9 @@ inline unsigned grid_for(int64_t items, int block) {
 }
 
 // Device error bits reported by kernels.
-enum : unsigned { DEV_BAD_TREE = 1u, DEV_STACK = 2u };
+// DEV_UNORDERED is informational: the tree's subtrees do not cover contiguous, ordered
+// leaf ranges (or are deeper than 64), so the rightmost leaves come from the climb.
+enum : unsigned { DEV_BAD_TREE = 1u, DEV_STACK = 2u, DEV_UNORDERED = 4u };
 
 // Per-batch-element counters kept on the device during a call.
 struct alignas(64) DevCounters {

@@ -21,8 +21,20 @@ namespace ccc {
 
 constexpr int kThreads = 256;
 constexpr int kTravThreads = 256;
-constexpr int kStackDepth = 24;   // pending right children per lane (tree depth - 1)
+constexpr int kLdsStack = 24;     // pending right children per lane kept in LDS
+constexpr int kStackDepth = 64;   // total per lane; entries past kLdsStack go to global memory
 constexpr int kStage = 256;       // staged pairs per wave before a flush
+
+// Workgroups are dispatched to the eight XCDs round-robin by workgroup ID, and each XCD has
+// its own L2.  Persistent kernels walk their tiles with CCC_FOR_TILES, which gives the
+// workgroups of one XCD one contiguous range of tiles (neighbouring tiles read the same
+// triangles); their grids are a multiple of kXcds.
+constexpr int kXcds = 8;
+#define CCC_FOR_TILES(tile, T)                                                                          \
+    for (int64_t tile##_k = blockIdx.x / kXcds, tile##_per = ((T) + kXcds - 1) / kXcds,                 \
+                 tile = (int64_t)(blockIdx.x % kXcds) * tile##_per + tile##_k;                          \
+         tile##_k < tile##_per && tile < (T);                                                           \
+         tile##_k += gridDim.x / kXcds, tile = (int64_t)(blockIdx.x % kXcds) * tile##_per + tile##_k)
 
 struct alignas(16) TNode {
     float4 a;  // L.min.x, L.min.y, L.min.z, L.max.x
@@ -118,12 +130,57 @@ __global__ void k_check_tree(const int64_t *__restrict__ tree_all, int64_t n, in
 // Rightmost leaf position under every internal node (build_bvh.cu:707-743): bottom-up
 // climb, the second thread to arrive at a node combines its children.  Bounded by n steps
 // (a longer climb can only come from a cycle, reported as DEV_BAD_TREE).
+// Rightmost leaf position of every internal node (build_bvh.cu:707-741 set_rightmost_nodes:
+// rightmost = the larger of the children's rightmost leaves, compared by leaf address).  Trees whose subtrees cover contiguous
+// ordered leaf ranges -- every tree bvh / bvh_motion builds -- take the walk below; any
+// other tree falls back to an atomic bottom-up climb.
+//
+// Walk: lo / hi of a node = position of the leaf reached by following left / right
+// children (at most 64 steps).
+__global__ void k_ranges(const int64_t *__restrict__ tree_all, int64_t n, int *__restrict__ lo_all, int *__restrict__ rm_all,
+                         DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (ctr[b].flags & DEV_BAD_TREE) return;
+    const int64_t *tree = tree_all + (int64_t)b * (2 * n - 1) * 4;
+    int64_t c = i, d = i;
+    for (int step = 0; c < n - 1 || d < n - 1; ++step) {
+        if (step >= 64) {
+            atomicOr(&ctr[b].flags, DEV_UNORDERED);
+            return;
+        }
+        if (c < n - 1) c = tree[4 * c + 1];
+        if (d < n - 1) d = tree[4 * d + 2];
+    }
+    lo_all[(int64_t)b * (n - 1) + i] = (int)(c - (n - 1));
+    rm_all[(int64_t)b * (n - 1) + i] = (int)(d - (n - 1));
+}
+
+// The walk's hi is the subtree maximum only if, at every node, the left child's range ends
+// right before the right child's begins.
+__global__ void k_order_check(const int64_t *__restrict__ tree_all, int64_t n, const int *__restrict__ lo_all,
+                              const int *__restrict__ rm_all, DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (ctr[b].flags & (DEV_BAD_TREE | DEV_UNORDERED)) return;
+    const int64_t *tree = tree_all + (int64_t)b * (2 * n - 1) * 4;
+    const int *lo = lo_all + (int64_t)b * (n - 1);
+    const int *rm = rm_all + (int64_t)b * (n - 1);
+    const int64_t l = tree[4 * i + 1], r = tree[4 * i + 2];
+    const int hl = l >= n - 1 ? (int)(l - (n - 1)) : rm[l];
+    const int lr = r >= n - 1 ? (int)(r - (n - 1)) : lo[r];
+    if (hl + 1 != lr) atomicOr(&ctr[b].flags, DEV_UNORDERED);
+}
+
+// Climb for the remaining trees: the second thread to reach a node combines its children.
 __global__ void k_rightmost(const int64_t *__restrict__ tree_all, int64_t n, int *__restrict__ counters_all,
                             int *__restrict__ rm_all, DevCounters *__restrict__ ctr) {
     const int b = blockIdx.y;
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= n || n < 2) return;
-    if (ctr[b].flags & DEV_BAD_TREE) return;
+    if ((ctr[b].flags & DEV_BAD_TREE) || !(ctr[b].flags & DEV_UNORDERED)) return;
     const int64_t rows = 2 * n - 1;
     const int64_t *tree = tree_all + (int64_t)b * rows * 4;
     int *counters = counters_all + (int64_t)b * (n - 1);
@@ -190,8 +247,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            const uint8_t *__restrict__ to_check_all,
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
-                                                           DevCounters *__restrict__ ctr_all) {
-    __shared__ int stack[kStackDepth][kTravThreads];
+                                                           DevCounters *__restrict__ ctr_all, int *__restrict__ spill_all) {
+    __shared__ int stack[kLdsStack][kTravThreads];
     __shared__ int2 stage[kTravThreads / 64][kStage + 128];
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
@@ -204,7 +261,12 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     if ((tid & 63) == 0) fill[w] = 0;
     __syncthreads();
 
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + tid;
+    // this lane's stack entries past kLdsStack (lane-interleaved, so a wave's spills coalesce)
+    const int64_t spill_stride = (int64_t)gridDim.x * gridDim.y * kTravThreads;
+    int *spill = spill_all + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * kTravThreads + tid;
+    // query blocks in XCD-contiguous order (the grid is a multiple of kXcds)
+    const int64_t qblock = (int64_t)(blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+    const int64_t p = qblock * blockDim.x + tid;
     bool active = p < n && n > 1 && !(ctr->flags & DEV_BAD_TREE);
     if (PARTIAL && active) active = to_check_all[(int64_t)b * n + p] != 0;
     const uint8_t *to_check = PARTIAL ? to_check_all + (int64_t)b * n : nullptr;
@@ -274,12 +336,15 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             const bool tl = ol && nd.d.x >= 0, tr = orr && nd.d.y >= 0;
             if (!tl && !tr) {
                 if (sp == 0) break;
-                node = stack[--sp][tid];
+                --sp;
+                node = sp < kLdsStack ? stack[sp][tid] : spill[(int64_t)(sp - kLdsStack) * spill_stride];
             } else {
                 node = tl ? nd.d.x : nd.d.y;
                 if (tl && tr) {
                     if (sp >= kStackDepth) { atomicOr(&ctr->flags, DEV_STACK); break; }
-                    stack[sp++][tid] = nd.d.y;
+                    if (sp < kLdsStack) stack[sp][tid] = nd.d.y;
+                    else spill[(int64_t)(sp - kLdsStack) * spill_stride] = nd.d.y;
+                    ++sp;
                 }
             }
         }
@@ -313,7 +378,8 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     const T *t0 = t0_all + (int64_t)b * n * 9;
     const T *t1 = MODE == 1 ? nullptr : t1_all + (int64_t)b * n * 9;
     int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
-    for (int64_t tile = blockIdx.x; c0 + tile * kThreads < K; tile += gridDim.x) {
+    const int64_t ntiles = K > c0 ? (K - c0 + kThreads - 1) / kThreads : 0;
+    CCC_FOR_TILES(tile, ntiles) {
         const int64_t i = c0 + tile * kThreads + threadIdx.x;
         uint32_t mask = 0;
         int x = 0, y = 0;
@@ -491,7 +557,11 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list;
 //   C  filterCollisions per root record (the tetrahedron is gathered again);
 //   D  (impulse mode) accepted collisions wait in an LDS stage; once a block's worth has
-//      gathered, every thread scatters one impulse.
+//      gathered, they are scattered through an LDS hash table (k_ccd keeps the scatter:
+//      a separate kernel over a global collision list measured twice as slow, the list
+//      mixing collisions of distant blocks).
+// The launch bound asks for five resident blocks per CU, which the collect mode reaches
+// (<= 102 VGPRs, 29 KB of LDS); the impulse mode stays at three (LDS).
 struct alignas(16) SolveRec {
     int64_t item;
     double a, b, c, d, f, g, h;
@@ -499,7 +569,7 @@ struct alignas(16) SolveRec {
 constexpr int kRootCap = 3 * kThreads;
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ t0_all, const T *__restrict__ t1_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
@@ -555,7 +625,8 @@ __global__ __launch_bounds__(kThreads) void k_ccd(const int64_t *__restrict__ it
         }
     };
 
-    for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
+    const int64_t ntiles = (M + kThreads - 1) / kThreads;
+    CCC_FOR_TILES(tile, ntiles) {
         // ---- A: coefficients and branch of one item per thread
         const int64_t i = tile * kThreads + tid;
         int code = CUBIC_NONE;
@@ -697,7 +768,8 @@ __global__ __launch_bounds__(kThreads) void k_proximity(const int64_t *__restric
     const int64_t *items = items_all + (int64_t)b * capM;
     const T *t0 = t0_all + (int64_t)b * n * 9;
     int64_t *out = out_all + (int64_t)b * out_cap * 3;
-    for (int64_t tile = blockIdx.x; tile * kThreads < M; tile += gridDim.x) {
+    const int64_t ntiles = (M + kThreads - 1) / kThreads;
+    CCC_FOR_TILES(tile, ntiles) {
         const int64_t i = tile * kThreads + threadIdx.x;
         int64_t x = 0, y = 0;
         int type = 0;
@@ -737,7 +809,8 @@ __global__ __launch_bounds__(kThreads) void k_sat(const int2 *__restrict__ pairs
     const int2 *pairs = pairs_all + (int64_t)b * capK;
     const T *t0 = t0_all + (int64_t)b * n * 9;
     int64_t *out = out_all + (int64_t)b * out_cap * 2;
-    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
+    const int64_t ntiles = (K + kThreads - 1) / kThreads;
+    CCC_FOR_TILES(tile, ntiles) {
         const int64_t i = tile * kThreads + threadIdx.x;
         unsigned keep = 0;
         int x = 0, y = 0;
@@ -783,7 +856,8 @@ __global__ __launch_bounds__(kThreads) void k_tri_edge(const int2 *__restrict__ 
     const T *t0 = t0_all + (int64_t)b * n * 9;
     constexpr int W = MODE == TE_CANDIDATES2 ? 2 : 4;
     int64_t *rows = rows_all + (int64_t)b * out_cap * W;
-    for (int64_t tile = blockIdx.x; tile * kThreads < K; tile += gridDim.x) {
+    const int64_t ntiles = (K + kThreads - 1) / kThreads;
+    CCC_FOR_TILES(tile, ntiles) {
         const int64_t i = tile * kThreads + threadIdx.x;
         int x = 0, y = 0;
         unsigned emit = 0;          // bit cid: row for edge cid
@@ -899,9 +973,9 @@ constexpr int64_t kPairChunk = (int64_t)1 << 25;
 static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B) {
     int64_t want = (items + kThreads - 1) / kThreads;
     int64_t limit = (int64_t)ctx->num_cus * 16 / (B > 0 ? B : 1);
-    if (limit < 4) limit = 4;
+    if (limit < kXcds) limit = kXcds;
     if (want > limit) want = limit;
-    if (want < 1) want = 1;
+    want = (want + kXcds - 1) / kXcds * kXcds;    // CCC_FOR_TILES needs a multiple of kXcds
     return (unsigned)want;
 }
 
@@ -916,6 +990,7 @@ static ccc_status run_query(const Query &q) {
     int *claims = ws.get_as<int>(B * n);
     int *climb = ws.get_as<int>(B * nint);
     int *rm = ws.get_as<int>(B * nint);
+    int *lo = ws.get_as<int>(B * nint);
     TNode *nodes = ws.get_as<TNode>(B * nint);
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
     ccc_status st;
@@ -954,6 +1029,10 @@ static ccc_status run_query(const Query &q) {
     const T *bb = (const T *)q.bboxes;
     tm.begin("tree_prep");
     hipLaunchKernelGGL(k_check_tree, dim3(grid_for(rows, 256), B), dim3(256), 0, s, q.tree, n, claims, ctr);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_ranges, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, q.tree, n, lo, rm, ctr);
+        hipLaunchKernelGGL(k_order_check, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, q.tree, n, lo, rm, ctr);
+    }
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
     tm.end();
@@ -964,16 +1043,21 @@ static ccc_status run_query(const Query &q) {
     int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
     bool warn_cand = false;
     int2 *pairs = nullptr;
+    const unsigned gT = (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
+    // spill area of the traversal stacks: (kStackDepth - kLdsStack) entries per lane; only
+    // trees deeper than kLdsStack + 1 levels touch it
+    int *spill = ws.get_as<int>((size_t)gT * B * kTravThreads * (kStackDepth - kLdsStack));
+    if (ws.failed()) return CCC_ERR_NO_MEMORY;
     for (int attempt = 0; attempt < 2; ++attempt) {
         pairs = ws.get_as<int2>(B * capK);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         tm.begin("traverse");
         if (q.to_check)
-            hipLaunchKernelGGL((k_traverse<T, true>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr);
+            hipLaunchKernelGGL((k_traverse<T, true>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
+                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
         else
-            hipLaunchKernelGGL((k_traverse<T, false>), dim3(grid_for(n, kTravThreads), B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr);
+            hipLaunchKernelGGL((k_traverse<T, false>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
+                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
@@ -1028,33 +1112,38 @@ static ccc_status run_query(const Query &q) {
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
-        for (int64_t c0 = 0; c0 < kmax; c0 += chunk) {
-            const int64_t c1 = c0 + chunk;
-            tm.begin("subprim");
-            if (q.narrow == Narrow::PROXIMITY)
-                hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
-                                   (const T *)nullptr, n, thr, items, capM, ctr);
-            else
-                hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
-                                   (const T *)q.t1, n, thr, items, capM, ctr);
-            tm.end();
-            CCC_LAUNCH_CHECK("k_subprim");
-            tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
-            if (q.narrow == Narrow::PROXIMITY)
-                hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
-                                   q.out_rows, q.out_cap, ctr);
-            else if (q.narrow == Narrow::CCD_COLLECT)
-                hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                                   (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap, (T *)nullptr,
-                                   (T *)nullptr, (int *)nullptr, ctr);
-            else
-                hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                                   (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
-                                   (T *)q.dv, (T *)q.dx, q.cnt, ctr);
-            tm.end();
-            hipLaunchKernelGGL(k_fold_items, dim3(1), dim3(64), 0, s, ctr, B);
-            CCC_LAUNCH_CHECK("narrow phase");
-        }
+        auto narrow_phase = [&]() -> ccc_status {
+            for (int64_t c0 = 0; c0 < kmax; c0 += chunk) {
+                const int64_t c1 = c0 + chunk;
+                tm.begin("subprim");
+                if (q.narrow == Narrow::PROXIMITY)
+                    hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
+                                       (const T *)nullptr, n, thr, items, capM, ctr);
+                else
+                    hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
+                                       (const T *)q.t1, n, thr, items, capM, ctr);
+                tm.end();
+                CCC_LAUNCH_CHECK("k_subprim");
+                tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
+                if (q.narrow == Narrow::PROXIMITY)
+                    hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
+                                       q.out_rows, q.out_cap, ctr);
+                else if (q.narrow == Narrow::CCD_COLLECT)
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                                       (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
+                                       (T *)nullptr, (T *)nullptr, (int *)nullptr, ctr);
+                else
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
+                                       (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
+                                       (T *)q.dv, (T *)q.dx, q.cnt, ctr);
+                tm.end();
+                CCC_LAUNCH_CHECK("narrow phase");
+                hipLaunchKernelGGL(k_fold_items, dim3(1), dim3(64), 0, s, ctr, B);
+                CCC_LAUNCH_CHECK("k_fold_items");
+            }
+            return CCC_OK;
+        };
+        if ((st = narrow_phase()) != CCC_OK) return st;
     }
     CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
     CCC_HIP_TRY(hipStreamSynchronize(s));

@@ -1,9 +1,10 @@
 // PyTorch-ROCm extension `cccollisions_amd._cccollisions`: the same 14 entry points,
 // argument names, defaults, output shapes and dtypes as the reference bindings
-// (src/cccollisions.cpp:102-468).  Each function checks its inputs, allocates the outputs
-// and calls the C ABI (include/cccollisions_amd.h) on the current HIP stream of the input's
-// device.  Errors raise RuntimeError (the reference exits the process); capacity overflows
-// raise a warning.
+// (src/cccollisions.cpp:102-468), plus compute_impulses_packed for meshes of different
+// sizes and a few controls (capacity policy, counts, kernel timing).
+// Each function checks its inputs, allocates the outputs and calls the C ABI
+// (include/cccollisions_amd.h) on the current HIP stream of the input's device.  Errors
+// raise RuntimeError (the reference exits the process); capacity overflows raise a warning.
 #include <ATen/DeviceGuard.h>
 #include <c10/hip/HIPCachingAllocator.h>
 #include <c10/hip/HIPStream.h>

@@ -34,3 +34,10 @@ def proximity_fold(n=24, seed=3, gap=1e-3, dtype=np.float64):
     """Static folded cloth: the two layers are `gap` apart, so the proximity list is not empty."""
     sc = scenes.folded_cloth(n, seed=seed, gap=gap)
     return sc.x0[sc.faces].astype(dtype)
+
+
+def garment(body_grid=8, rings=24, segments=23, seed=3, dtype=np.float64):
+    """Small garment-on-body scene: static capsule body + a sleeve moving inward."""
+    sc = scenes.garment_on_body(body_grid, rings, segments, seed=seed)
+    t0, t1 = sc.soups()
+    return t0.astype(dtype), t1.astype(dtype), sc.corner_masses().astype(dtype)

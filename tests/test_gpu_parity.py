@@ -31,6 +31,7 @@ def _check_close(gpu, ref, name):
 
 SCENES = {
     "folded20": lambda: cases.folded(20),
+    "garment": lambda: cases.garment(),
     "random300": lambda: cases.random_soup(300),
     "shared200": lambda: cases.shared_vertex_soup(200),
 }
@@ -377,6 +378,52 @@ def test_static_pairs_have_no_roots(ccc, oracle):
     K, M, V, R = ccc.last_counts()[0]
     assert K > 0 and M > 0 and V == 0 and R == 0
     assert (_np(gc) == -1).all() and (_np(gr) == -1).all()
+
+
+def _shuffled_leaf_tree(rb, rt, seed):
+    """A valid wire tree over the same leaves whose subtrees no longer cover contiguous
+    position ranges: leaf rows are permuted (each keeps its box and triangle id, its parent
+    now points at the new row) and the internal boxes are refit."""
+    n = (rt.shape[0] + 1) // 2
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)                       # leaf row n-1+k moves to row n-1+perm[k]
+    bb, tree = rb.copy(), rt.copy()
+    for k in range(n):
+        src, dst = n - 1 + k, n - 1 + perm[k]
+        bb[dst], tree[dst] = rb[src], rt[src]
+    for i in range(n - 1):
+        for c in (1, 2):
+            if rt[i, c] >= n - 1:
+                tree[i, c] = n - 1 + perm[rt[i, c] - (n - 1)]
+    order = sorted(range(n - 1), key=lambda i: -_depth(tree, i))
+    for i in order:
+        l, r = tree[i, 1], tree[i, 2]
+        bb[i, :3] = np.minimum(bb[l, :3], bb[r, :3])
+        bb[i, 3:] = np.maximum(bb[l, 3:], bb[r, 3:])
+    return bb, tree
+
+
+def _depth(tree, i):
+    d = 0
+    while tree[i, 0] >= 0:
+        i, d = tree[i, 0], d + 1
+    return d
+
+
+def test_unordered_tree_takes_the_climb(ccc, oracle):
+    """Trees from other builders need not cover contiguous ranges; the rightmost leaves then
+    come from the atomic climb and the pairs still equal the oracle's on the same tree."""
+    t0, t1, _ = cases.random_soup(300, seed=13)
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    bb, tree = _shuffled_leaf_tree(rb, rt, 3)
+    pairs, _ = oracle.traverse(bb, tree, 0.0)
+    ref_pairs, _ = oracle.traverse(rb, rt, 0.0)
+    np.testing.assert_array_equal(sorted_rows(pairs), sorted_rows(ref_pairs))   # topology independent
+    coll, roots, counts = oracle.ccd(t0, t1, pairs)
+    mc = int(np.ceil(counts["candidates"] / 300)) + 1
+    gc, gr = ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), mc, 64)
+    assert ccc.last_counts()[0][0] == counts["candidates"]
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
 
 
 def test_bad_tree_raises(ccc, oracle):
