@@ -236,6 +236,40 @@ __global__ void k_nodes(const T *__restrict__ bb_all, const int64_t *__restrict_
     nodes_all[(int64_t)b * (n - 1) + i] = nd;
 }
 
+// Leaf record by position: the leaf box rounded inward to fp32 and the leaf's triangle id.
+// A query box that overlaps the inward box overlaps the exact box (fp rounding is monotone,
+// so fl(inward.max + thr) <= fl(max + thr) and fl(inward.min - thr) >= fl(min - thr)); only
+// when the inward test fails does the traversal read the exact box from `bboxes`.
+struct alignas(32) LeafRec {
+    float mn[3], mx[3];
+    int id, pad;
+};
+
+template <typename T>
+__global__ void k_leaf_recs(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all, int64_t n,
+                            LeafRec *__restrict__ recs_all, const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (ctr[b].flags & DEV_BAD_TREE) return;
+    const int64_t rows = 2 * n - 1, row = n - 1 + p;
+    const T *bx = bb_all + (int64_t)b * rows * 6 + 6 * row;
+    LeafRec r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = f32_up((double)bx[k]);
+        r.mx[k] = f32_down((double)bx[k + 3]);
+    }
+    r.id = (int)tree_all[(int64_t)b * rows * 4 + 4 * row + 3];
+    r.pad = 0;
+    recs_all[(int64_t)b * n + p] = r;
+}
+
+template <typename T> __device__ __forceinline__ bool overlap_inward(const Box<T> &q, const LeafRec &r, T thr) {
+    return (q.mn.x <= ((T)r.mx[0] + thr)) && (q.mx.x >= ((T)r.mn[0] - thr)) && (q.mn.y <= ((T)r.mx[1] + thr)) &&
+           (q.mx.y >= ((T)r.mn[1] - thr)) && (q.mn.z <= ((T)r.mx[2] + thr)) && (q.mx.z >= ((T)r.mn[2] - thr));
+}
+
 // ---- traversal ----------------------------------------------------------------------------
 // collisions_static.cu:10-120 (ordering on) and collisions_continuous.cu:127-230,
 // :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
@@ -243,7 +277,8 @@ __global__ void k_nodes(const T *__restrict__ bb_all, const int64_t *__restrict_
 // the boxes stored in `bboxes`; the pair is (min id, max id).
 template <typename T, bool PARTIAL>
 __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
-                                                           const TNode *__restrict__ nodes_all, int64_t n, T thr,
+                                                           const TNode *__restrict__ nodes_all,
+                                                           const LeafRec *__restrict__ recs_all, int64_t n, T thr,
                                                            const uint8_t *__restrict__ to_check_all,
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
@@ -256,6 +291,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     const T *bb = bb_all + (int64_t)b * rows * 6;
     const int64_t *tree = tree_all + (int64_t)b * rows * 4;
     const TNode *nodes = nodes_all + (int64_t)b * (n - 1);
+    const LeafRec *recs = recs_all + (int64_t)b * n;
     DevCounters *ctr = ctr_all + b;
     int2 *pairs = pairs_all + (int64_t)b * cap;
     if ((tid & 63) == 0) fill[w] = 0;
@@ -274,7 +310,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     if (active) {
         const int64_t qrow = n - 1 + p;
         const Box<T> q = ld_box(bb + 6 * qrow);
-        const int qid = (int)tree[4 * qrow + 3];
+        const int qid = recs[p].id;
         const float qx0 = f32_down((double)q.mn.x), qy0 = f32_down((double)q.mn.y), qz0 = f32_down((double)q.mn.z);
         const float qx1 = f32_up((double)q.mx.x), qy1 = f32_up((double)q.mx.y), qz1 = f32_up((double)q.mx.z);
         // packed meshes: leaves at positions >= lim belong to later meshes
@@ -294,16 +330,16 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             orr = orr && nd.d.z + 1 < lim;
             int oid0 = -1, oid1 = -1;
             if (ol && nd.d.x < 0) {
-                const int64_t lrow = n - 1 + (int64_t)(~nd.d.x);
-                if (overlap(q, ld_box(bb + 6 * lrow), thr)) {
-                    const int oid = (int)tree[4 * lrow + 3];
+                const LeafRec lr = recs[~nd.d.x];
+                if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~nd.d.x))), thr)) {
+                    const int oid = lr.id;
                     if (!PARTIAL || !to_check[oid] || qid < oid) oid0 = oid;
                 }
             }
             if (orr && nd.d.y < 0) {
-                const int64_t rrow = n - 1 + (int64_t)(~nd.d.y);
-                if (overlap(q, ld_box(bb + 6 * rrow), thr)) {
-                    const int oid = (int)tree[4 * rrow + 3];
+                const LeafRec lr = recs[~nd.d.y];
+                if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~nd.d.y))), thr)) {
+                    const int oid = lr.id;
                     if (!PARTIAL || !to_check[oid] || qid < oid) oid1 = oid;
                 }
             }
@@ -557,11 +593,13 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list;
 //   C  filterCollisions per root record (the tetrahedron is gathered again);
 //   D  (impulse mode) accepted collisions wait in an LDS stage; once a block's worth has
-//      gathered, they are scattered through an LDS hash table (k_ccd keeps the scatter:
-//      a separate kernel over a global collision list measured twice as slow, the list
-//      mixing collisions of distant blocks).
+//      gathered, they are scattered through an LDS hash table.  The scatter stays in this
+//      kernel: run as a kernel of its own over per-workgroup collision lists it took 14 ms
+//      per pass on the 130k cloth against ~6 here, where other workgroups' fp64 work hides
+//      its atomic latency (and that more than ate the 5-wave occupancy the CCD phases get
+//      without the scatter's registers).
 // The launch bound asks for five resident blocks per CU, which the collect mode reaches
-// (<= 102 VGPRs, 29 KB of LDS); the impulse mode stays at three (LDS).
+// (<= 102 VGPRs, 29 KB of LDS); the impulse mode stays at three (157 VGPRs, 45 KB of LDS).
 struct alignas(16) SolveRec {
     int64_t item;
     double a, b, c, d, f, g, h;
@@ -992,6 +1030,7 @@ static ccc_status run_query(const Query &q) {
     int *rm = ws.get_as<int>(B * nint);
     int *lo = ws.get_as<int>(B * nint);
     TNode *nodes = ws.get_as<TNode>(B * nint);
+    LeafRec *recs = ws.get_as<LeafRec>(B * n);
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
     ccc_status st;
     if ((st = fill_u64(ctr, B * (int64_t)(sizeof(DevCounters) / 8), 0ull, s)) != CCC_OK) return st;
@@ -1035,6 +1074,7 @@ static ccc_status run_query(const Query &q) {
     }
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
+    hipLaunchKernelGGL((k_leaf_recs<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, bb, q.tree, n, recs, ctr);
     tm.end();
     CCC_LAUNCH_CHECK("tree preparation");
 
@@ -1054,10 +1094,10 @@ static ccc_status run_query(const Query &q) {
         tm.begin("traverse");
         if (q.to_check)
             hipLaunchKernelGGL((k_traverse<T, true>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
+                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
         else
             hipLaunchKernelGGL((k_traverse<T, false>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
+                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
@@ -1109,9 +1149,9 @@ static ccc_status run_query(const Query &q) {
         const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, kPairChunk), 1);
         const int64_t capM = 15 * chunk;
         int64_t *items = ws.get_as<int64_t>(B * capM);
-        if (ws.failed()) return CCC_ERR_NO_MEMORY;
         const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
+        if (ws.failed()) return CCC_ERR_NO_MEMORY;
         auto narrow_phase = [&]() -> ccc_status {
             for (int64_t c0 = 0; c0 < kmax; c0 += chunk) {
                 const int64_t c1 = c0 + chunk;
