@@ -21,9 +21,10 @@ namespace ccc {
 
 constexpr int kThreads = 256;
 constexpr int kTravThreads = 256;
-constexpr int kLdsStack = 24;     // pending right children per lane kept in LDS
-constexpr int kStackDepth = 64;   // total per lane; entries past kLdsStack go to global memory
+constexpr int kStackDepth = 32;   // pending right children per lane (LDS)
 constexpr int kStage = 256;       // staged pairs per wave before a flush
+constexpr int kFrontierDepth = 3; // queries start from the subtrees this deep below the root
+constexpr int kFrontier = 1 << kFrontierDepth;
 
 // Workgroups are dispatched to the eight XCDs round-robin by workgroup ID, and each XCD has
 // its own L2.  Persistent kernels walk their tiles with CCC_FOR_TILES, which gives the
@@ -270,6 +271,64 @@ template <typename T> __device__ __forceinline__ bool overlap_inward(const Box<T
            (q.mx.y >= ((T)r.mn[1] - thr)) && (q.mn.z <= ((T)r.mx[2] + thr)) && (q.mx.z >= ((T)r.mn[2] - thr));
 }
 
+// Start points of the traversal: the subtrees kFrontierDepth levels below the root (a leaf
+// reached earlier stands for itself), each with its fp32 box grown outward by the
+// threshold (as in TNode), its rightmost leaf and its first leaf position.  A query runs
+// one traversal per start point, so the launch has kFrontier times as many lanes.
+struct Frontier {
+    int count;
+    int ref[kFrontier];      // internal node index, or ~leaf position
+    int rightmost[kFrontier];
+    int first[kFrontier];    // first leaf position (in-order trees; 0 otherwise)
+    float box[kFrontier][6];
+};
+
+template <typename T>
+__global__ void k_frontier(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all, const int *__restrict__ rm_all,
+                           const int *__restrict__ lo_all, int64_t n, double thr, Frontier *__restrict__ fr_all,
+                           const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    Frontier f;
+    f.count = 0;
+    if (n < 2 || (ctr[b].flags & DEV_BAD_TREE)) {
+        fr_all[b] = f;
+        return;
+    }
+    const int64_t rows = 2 * n - 1;
+    const T *bb = bb_all + (int64_t)b * rows * 6;
+    const int64_t *tree = tree_all + (int64_t)b * rows * 4;
+    const int *rm = rm_all + (int64_t)b * (n - 1);
+    const int *lo = lo_all + (int64_t)b * (n - 1);
+    const bool ordered = !(ctr[b].flags & DEV_UNORDERED);
+    int cur[kFrontier], nc = 1;
+    cur[0] = 0;
+    for (int level = 0; level < kFrontierDepth; ++level) {
+        int next[kFrontier], nn = 0;
+        for (int i = 0; i < nc; ++i) {
+            if (cur[i] < 0) { next[nn++] = cur[i]; continue; }
+            const int64_t l = tree[4 * cur[i] + 1], r = tree[4 * cur[i] + 2];
+            next[nn++] = l >= n - 1 ? ~(int)(l - (n - 1)) : (int)l;
+            next[nn++] = r >= n - 1 ? ~(int)(r - (n - 1)) : (int)r;
+        }
+        for (int i = 0; i < nn; ++i) cur[i] = next[i];
+        nc = nn;
+    }
+    f.count = nc;
+    for (int i = 0; i < nc; ++i) {
+        const int ref = cur[i];
+        const int64_t row = ref >= 0 ? ref : n - 1 + (int64_t)(~ref);
+        f.ref[i] = ref;
+        f.rightmost[i] = ref >= 0 ? rm[ref] : ~ref;
+        f.first[i] = ref >= 0 ? (ordered ? lo[ref] : 0) : ~ref;
+        for (int k = 0; k < 3; ++k) {
+            f.box[i][k] = f32_down((double)bb[6 * row + k] - thr);
+            f.box[i][k + 3] = f32_up((double)bb[6 * row + k + 3] + thr);
+        }
+    }
+    fr_all[b] = f;
+}
+
 // ---- traversal ----------------------------------------------------------------------------
 // collisions_static.cu:10-120 (ordering on) and collisions_continuous.cu:127-230,
 // :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
@@ -282,8 +341,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            const uint8_t *__restrict__ to_check_all,
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
-                                                           DevCounters *__restrict__ ctr_all, int *__restrict__ spill_all) {
-    __shared__ int stack[kLdsStack][kTravThreads];
+                                                           DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all) {
+    __shared__ int stack[kStackDepth][kTravThreads];
     __shared__ int2 stage[kTravThreads / 64][kStage + 128];
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
@@ -297,13 +356,14 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     if ((tid & 63) == 0) fill[w] = 0;
     __syncthreads();
 
-    // this lane's stack entries past kLdsStack (lane-interleaved, so a wave's spills coalesce)
-    const int64_t spill_stride = (int64_t)gridDim.x * gridDim.y * kTravThreads;
-    int *spill = spill_all + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * kTravThreads + tid;
-    // query blocks in XCD-contiguous order (the grid is a multiple of kXcds)
-    const int64_t qblock = (int64_t)(blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+    // block -> (start point s, query block); query blocks in XCD-contiguous order (the
+    // query-block count nqb is a multiple of kXcds)
+    const int nqb = gridDim.x / kFrontier;
+    const int s = blockIdx.x / nqb, local = blockIdx.x % nqb;
+    const int64_t qblock = (int64_t)(local % kXcds) * (nqb / kXcds) + local / kXcds;
     const int64_t p = qblock * blockDim.x + tid;
-    bool active = p < n && n > 1 && !(ctr->flags & DEV_BAD_TREE);
+    const Frontier &fr = fr_all[b];
+    bool active = p < n && n > 1 && !(ctr->flags & DEV_BAD_TREE) && s < fr.count;
     if (PARTIAL && active) active = to_check_all[(int64_t)b * n + p] != 0;
     const uint8_t *to_check = PARTIAL ? to_check_all + (int64_t)b * n : nullptr;
 
@@ -315,9 +375,23 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
         const float qx1 = f32_up((double)q.mx.x), qy1 = f32_up((double)q.mx.y), qz1 = f32_up((double)q.mx.z);
         // packed meshes: leaves at positions >= lim belong to later meshes
         const int lim = mesh_end ? mesh_end[p] : INT_MAX;
-        int sp = 0, node = 0;
-        while (true) {
-            const TNode nd = nodes[node];
+        // the start point itself: box, rightmost and first-leaf tests as for a child
+        const float *fb = fr.box[s];
+        bool enter = qx0 <= fb[3] && qx1 >= fb[0] && qy0 <= fb[4] && qy1 >= fb[1] && qz0 <= fb[5] && qz1 >= fb[2];
+        if (!PARTIAL) enter = enter && fr.rightmost[s] > (int)p;
+        enter = enter && fr.first[s] < lim;
+        int sp = 0, node = fr.ref[s];
+        while (enter) {
+            TNode nd;
+            if (node >= 0) {
+                nd = nodes[node];
+            } else {
+                // a start point that is a leaf: examined once as a left child that passed its tests
+                nd.a = make_float4(-INFINITY, -INFINITY, -INFINITY, INFINITY);
+                nd.b = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+                nd.c = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+                nd.d = make_int4(node, 0, ~node, INT_MIN);
+            }
             bool ol = qx0 <= nd.a.w && qx1 >= nd.a.x && qy0 <= nd.b.x && qy1 >= nd.a.y && qz0 <= nd.b.y && qz1 >= nd.a.z;
             bool orr = qx0 <= nd.c.y && qx1 >= nd.b.z && qy0 <= nd.c.z && qy1 >= nd.b.w && qz0 <= nd.c.w && qz1 >= nd.c.x;
             if (!PARTIAL) {
@@ -373,14 +447,12 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             if (!tl && !tr) {
                 if (sp == 0) break;
                 --sp;
-                node = sp < kLdsStack ? stack[sp][tid] : spill[(int64_t)(sp - kLdsStack) * spill_stride];
+                node = stack[sp][tid];
             } else {
                 node = tl ? nd.d.x : nd.d.y;
                 if (tl && tr) {
                     if (sp >= kStackDepth) { atomicOr(&ctr->flags, DEV_STACK); break; }
-                    if (sp < kLdsStack) stack[sp][tid] = nd.d.y;
-                    else spill[(int64_t)(sp - kLdsStack) * spill_stride] = nd.d.y;
-                    ++sp;
+                    stack[sp++][tid] = nd.d.y;
                 }
             }
         }
@@ -1031,6 +1103,7 @@ static ccc_status run_query(const Query &q) {
     int *lo = ws.get_as<int>(B * nint);
     TNode *nodes = ws.get_as<TNode>(B * nint);
     LeafRec *recs = ws.get_as<LeafRec>(B * n);
+    Frontier *frontier = ws.get_as<Frontier>(B);
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
     ccc_status st;
     if ((st = fill_u64(ctr, B * (int64_t)(sizeof(DevCounters) / 8), 0ull, s)) != CCC_OK) return st;
@@ -1075,6 +1148,7 @@ static ccc_status run_query(const Query &q) {
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
     hipLaunchKernelGGL((k_leaf_recs<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, bb, q.tree, n, recs, ctr);
+    hipLaunchKernelGGL((k_frontier<T>), dim3(B), dim3(64), 0, s, bb, q.tree, rm, lo, n, q.thr, frontier, ctr);
     tm.end();
     CCC_LAUNCH_CHECK("tree preparation");
 
@@ -1083,21 +1157,18 @@ static ccc_status run_query(const Query &q) {
     int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
     bool warn_cand = false;
     int2 *pairs = nullptr;
-    const unsigned gT = (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
-    // spill area of the traversal stacks: (kStackDepth - kLdsStack) entries per lane; only
-    // trees deeper than kLdsStack + 1 levels touch it
-    int *spill = ws.get_as<int>((size_t)gT * B * kTravThreads * (kStackDepth - kLdsStack));
-    if (ws.failed()) return CCC_ERR_NO_MEMORY;
+    // one block per (start point, query block); query blocks rounded to a multiple of kXcds
+    const unsigned gT = kFrontier * (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
     for (int attempt = 0; attempt < 2; ++attempt) {
         pairs = ws.get_as<int2>(B * capK);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         tm.begin("traverse");
         if (q.to_check)
             hipLaunchKernelGGL((k_traverse<T, true>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
+                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier);
         else
             hipLaunchKernelGGL((k_traverse<T, false>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, spill);
+                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier);
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));

@@ -36,16 +36,15 @@ class Request:
 
 def footprint_bytes(num_triangles: int, max_candidates: int, itemsize: int = 8) -> int:
     """HBM a one-mesh step needs beyond the resident state: the two soups, the BVH and its
-    traversal nodes, the traversal stacks' spill area, the candidate list at capacity, one
-    chunk of sub-primitive items and the impulse outputs."""
+    traversal nodes and leaf records, the candidate list at capacity, one chunk of
+    sub-primitive items and the impulse outputs."""
     n = num_triangles
     soups = 2 * n * 9 * itemsize
-    bvh = (2 * n - 1) * (6 * itemsize + 4 * 8) + (n - 1) * 64
-    spill = n * 40 * 4
+    bvh = (2 * n - 1) * (6 * itemsize + 4 * 8) + (n - 1) * 64 + n * 32
     pairs = max_candidates * n * 8
     items = 15 * min(max_candidates * n, 1 << 25) * 8
     outs = n * (18 * itemsize + 12)
-    return soups + bvh + spill + pairs + items + outs
+    return soups + bvh + pairs + items + outs
 
 
 class Scheduler:
