@@ -40,14 +40,14 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline runs every n-th query leaf of one pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batched-sims", type=int, default=0,
+    ap.add_argument("--batched-sims", type=int, default=64,
                     help="also time the packed multi-mesh path against the plain per-mesh path on this many "
                          "folded cloths of 2k-200k triangles (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=1)
-    ap.add_argument("--garment", action="store_true",
-                    help="also time the garment-on-body scene (static 24k-triangle capsule body + 100k-triangle "
+    ap.add_argument("--no-garment", dest="garment", action="store_false",
+                    help="skip the garment-on-body scene (static 24k-triangle capsule body + 100k-triangle "
                          "sleeve moving inward)")
-    ap.add_argument("--serving-sims", type=int, default=0,
+    ap.add_argument("--serving-sims", type=int, default=32,
                     help="also run the serving loop over this many resident folded cloths (lognormal sizes, "
                          "median 20k triangles; 0: skip)")
     ap.add_argument("--serving-max-tris", type=int, default=200000)
@@ -55,10 +55,15 @@ def parse():
     ap.add_argument("--serving-slots", type=int, default=2)
     ap.add_argument("--serving-load", type=float, default=0.7,
                     help="offered load of the open-loop run as a fraction of the saturation rate")
+    ap.add_argument("--main-only", action="store_true",
+                    help="time only the 130k-triangle cloth (no garment, batched or serving section)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy dispatch of "
                          "known size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.main_only:
+        args.garment, args.batched_sims, args.serving_sims = False, 0, 0
+    return args
 
 
 class Step:

@@ -366,13 +366,41 @@ template <typename T> __device__ __forceinline__ void tet_at(const MTet<T> &m, T
     for (int k = 0; k < 4; ++k) p[k] = m.v[k] + m.dv[k] * t;
 }
 
-// collisions_continuous.cu:615-637 is_inside (NaN passes every comparison: accepted).
+// Where the correctly rounded quotient RN(n / d) lies, decided without dividing when that
+// is certain: -1 below 0, +1 above 1, 0 when it may lie in [0, 1] (or when d is not a
+// finite number >= dmin, or n is not finite) -- the caller then divides.  With e = 2^-49
+// (double) / 2^-20 (float) and unit roundoff u: d * e is exact for d >= dmin, so
+// n < -d e gives n / d < -e and RN(n / d) <= -e < 0; and n > RN(d + d e) >= d (1 + e)(1 - u)
+// > d (1 + e / 2) gives RN(n / d) >= 1 + e / 2 > 1 (rounding is monotone).  Used only where
+// the quotient feeds nothing but these comparisons, so results are unchanged.
+__device__ __forceinline__ int ratio_side(double n, double d) {
+    constexpr double e = 0x1p-49, dmin = 1e-280, big = 1.7976931348623157e308;
+    if (!(d >= dmin && d <= big && fabs(n) <= big)) return 0;
+    const double de = d * e;
+    if (n < -de) return -1;
+    if (n > d + de) return 1;
+    return 0;
+}
+__device__ __forceinline__ int ratio_side(float n, float d) {
+    constexpr float e = 0x1p-20f, dmin = 1e-30f, big = 3.40282347e38f;
+    if (!(d >= dmin && d <= big && fabsf(n) <= big)) return 0;
+    const float de = d * e;
+    if (n < -de) return -1;
+    if (n > d + de) return 1;
+    return 0;
+}
+
+// collisions_continuous.cu:615-637 is_inside (NaN passes every comparison: accepted).  A
+// barycentric coordinate that is certainly outside [0, 1] rejects before any division
+// (the reference rejects too, at that check or an earlier one).
 template <typename T> __device__ __forceinline__ bool is_inside(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> vt) {
     V3<T> e1 = v0 - v1, e2 = v2 - v1, ep = vt - v1;
     T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
     T denom = d00 * d11 - d01 * d01;
-    T v = (d11 * d20 - d01 * d21) / denom;
-    T w = (d00 * d21 - d01 * d20) / denom;
+    const T nv = d11 * d20 - d01 * d21, nw = d00 * d21 - d01 * d20;
+    if (ratio_side(nv, denom) != 0 || ratio_side(nw, denom) != 0) return false;
+    T v = nv / denom;
+    T w = nw / denom;
     T u = 1 - v - w;
     if (v < 0 || v > 1) return false;
     if (w < 0 || w > 1) return false;
@@ -388,8 +416,11 @@ template <typename T> __device__ __forceinline__ bool filter_collision(const MTe
     if (type < 9) {
         V3<T> da = p[1] - p[0], db = p[3] - p[2], dc = p[2] - p[0];
         V3<T> cab = cross(da, db);
-        T s = dot(cross(dc, db), cab) / norm2(cab);
-        T q = dot(cross(dc, da), cab) / norm2(cab);
+        const T ns = dot(cross(dc, db), cab), nq = dot(cross(dc, da), cab), den = norm2(cab);
+        // a parameter certainly outside [0, 1] rejects without dividing (same result)
+        if (ratio_side(ns, den) != 0 || ratio_side(nq, den) != 0) return false;
+        T s = ns / den;
+        T q = nq / den;
         return !(s < 0 || s > 1 || q < 0 || q > 1);
     } else if (type < 12) {
         return is_inside(p[0], p[1], p[2], p[3]);
@@ -407,21 +438,42 @@ template <typename T> struct ImpulseParams {
     T im0, im1, keep;
 };
 
+// The two weights of the collision point: edge-edge (type < 9) the closest-point
+// parameters s, t of the two edges; vertex-face the barycentric u, w of the face.
+template <typename T> struct ImpulseWeights { T a, b; };
+
 template <typename T>
-__device__ __forceinline__ ImpulseParams<T> collision_impulse(const MTet<T> &m, int type, T root, const T *q) {
+__device__ __forceinline__ ImpulseWeights<T> impulse_weights(const MTet<T> &m, int type, T root) {
     V3<T> p[4];
     tet_at(m, root, p);
-    V3<T> nrm;
-    T nvel, mpx, mpy;
+    ImpulseWeights<T> w;
     if (type < 9) {
         V3<T> da = p[1] - p[0], db = p[3] - p[2], dc = p[2] - p[0];
         T denom = norm2(cross(da, db));
-        T s, tt;
-        if (tabs(denom) < 1e-15) { s = (T)0.5; tt = (T)0.5; }
+        if (tabs(denom) < 1e-15) { w.a = (T)0.5; w.b = (T)0.5; }
         else {
-            s = dot(cross(dc, db), cross(da, db)) / norm2(cross(da, db));
-            tt = dot(cross(dc, da), cross(da, db)) / norm2(cross(da, db));
+            w.a = dot(cross(dc, db), cross(da, db)) / norm2(cross(da, db));
+            w.b = dot(cross(dc, da), cross(da, db)) / norm2(cross(da, db));
         }
+    } else {
+        const bool inv = type >= 12;
+        const int i0 = inv ? 1 : 0, i1 = inv ? 2 : 1, i2 = inv ? 3 : 2, it = inv ? 0 : 3;
+        V3<T> e1 = p[i0] - p[i1], e2 = p[i2] - p[i1], ep = p[it] - p[i1];
+        T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
+        T denom = d00 * d11 - d01 * d01;
+        if (tabs(denom) < 1e-15) { w.a = (T)(1. / 3.); w.b = (T)(1. / 3.); }
+        else { w.a = (d11 * d20 - d01 * d21) / denom; w.b = (d00 * d21 - d01 * d20) / denom; }
+    }
+    return w;
+}
+
+template <typename T>
+__device__ __forceinline__ ImpulseParams<T> impulse_params(const MTet<T> &m, int type, T root, ImpulseWeights<T> w,
+                                                           const T *q) {
+    V3<T> nrm;
+    T nvel, mpx, mpy;
+    if (type < 9) {
+        const T s = w.a, tt = w.b;
         V3<T> p0 = m.v[0] + (m.v[1] - m.v[0]) * s;
         V3<T> p1 = m.v[2] + (m.v[3] - m.v[2]) * tt;
         V3<T> dp0 = m.dv[0] + (m.dv[1] - m.dv[0]) * s;
@@ -435,22 +487,17 @@ __device__ __forceinline__ ImpulseParams<T> collision_impulse(const MTet<T> &m, 
     } else {
         const bool inv = type >= 12;
         const int i0 = inv ? 1 : 0, i1 = inv ? 2 : 1, i2 = inv ? 3 : 2, it = inv ? 0 : 3;
-        V3<T> e1 = p[i0] - p[i1], e2 = p[i2] - p[i1], ep = p[it] - p[i1];
-        T d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(ep, e1), d21 = dot(ep, e2);
-        T denom = d00 * d11 - d01 * d01;
-        T u, w;
-        if (tabs(denom) < 1e-15) { u = (T)(1. / 3.); w = (T)(1. / 3.); }
-        else { u = (d11 * d20 - d01 * d21) / denom; w = (d00 * d21 - d01 * d20) / denom; }
-        T v = 1 - u - w;
-        V3<T> pt = m.v[i0] * u + m.v[i1] * v + m.v[i2] * w;
-        V3<T> dpt = m.dv[i0] * u + m.dv[i1] * v + m.dv[i2] * w;
+        const T u = w.a, ww = w.b;
+        T v = 1 - u - ww;
+        V3<T> pt = m.v[i0] * u + m.v[i1] * v + m.v[i2] * ww;
+        V3<T> dpt = m.dv[i0] * u + m.dv[i1] * v + m.dv[i2] * ww;
         V3<T> c0 = m.v[i0] + m.dv[i0] * root, c1 = m.v[i1] + m.dv[i1] * root, c2 = m.v[i2] + m.dv[i2] * root;
         nrm = normalize(cross(c1 - c0, c2 - c0));
         if (dot(m.v[it] - pt, nrm) < 0) nrm = nrm * (T)-1;
         nvel = dot(m.dv[it] - dpt, nrm);
         if (inv) nrm = nrm * (T)-1;
-        if (inv) { mpx = q[0]; mpy = q[1] * u + q[2] * v + q[3] * w; }
-        else { mpx = q[0] * u + q[1] * v + q[2] * w; mpy = q[3]; }
+        if (inv) { mpx = q[0]; mpy = q[1] * u + q[2] * v + q[3] * ww; }
+        else { mpx = q[0] * u + q[1] * v + q[2] * ww; mpy = q[3]; }
     }
     T offset;
     if (tabs(nvel) < 1e-15) offset = (T)0.1;
@@ -467,6 +514,11 @@ __device__ __forceinline__ ImpulseParams<T> collision_impulse(const MTet<T> &m, 
     const T root_offset = (T)fmax((double)(root - offset), 0.);
     ip.keep = 1 - root_offset;
     return ip;
+}
+
+template <typename T>
+__device__ __forceinline__ ImpulseParams<T> collision_impulse(const MTet<T> &m, int type, T root, const T *q) {
+    return impulse_params(m, type, root, impulse_weights(m, type, root), q);
 }
 
 // Impulse of tetrahedron corner k: n * imp0 for the first primitive's corners, n * imp1 for

@@ -533,7 +533,10 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
 // The tetrahedron is gathered again for the filter so that it is not live across the
 // cubic solver (fewer VGPRs, more waves per SIMD).
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
-constexpr int kImpStage = 4 * kThreads;   // a tile adds at most 3 * kThreads to < kThreads staged
+// Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
+// tiles and a tile adds at most 3 * kThreads.
+constexpr int kFlush = kThreads / 4;
+constexpr int kImpStage = kFlush + 3 * kThreads;
 
 template <typename T>
 __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
@@ -564,8 +567,10 @@ __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T ro
 // Per-block sums of corner impulses: open addressing on the corner key 3 * triangle +
 // corner, kImpSlots slots (a power of two), linear probing up to kImpProbe slots; a
 // contribution that finds no slot goes to the global arrays directly.
-constexpr int kImpSlots = 512;
+constexpr int kImpSlots = 2 * kFlush;
 constexpr int kImpProbe = 16;
+constexpr int log2_exact(int v) { return v <= 1 ? 0 : 1 + log2_exact(v / 2); }
+constexpr int kImpBits = log2_exact(kImpSlots);
 
 template <typename T> struct ImpTable {
     int *key;
@@ -590,13 +595,35 @@ template <typename T> struct ImpTable {
     }
     // slot of `k`, inserting it if absent; -1 when the probe window is full
     __device__ int slot(int k) {
-        int h = (int)(((unsigned)k * 2654435761u) >> (32 - 9)) & (kImpSlots - 1);
+        int h = (int)(((unsigned)k * 2654435761u) >> (32 - kImpBits)) & (kImpSlots - 1);
         for (int p = 0; p < kImpProbe; ++p) {
             const int old = atomicCAS(&key[h], -1, k);
             if (old == -1 || old == k) return h;
             h = (h + 1) & (kImpSlots - 1);
         }
         return -1;
+    }
+    // adds one corner's impulse and position correction; to the global arrays directly
+    // when the probe window is full
+    __device__ void add(int64_t c, const V3<T> &imp, const V3<T> &imd, T *dv, T *dx, int *cnt) {
+        const int s = slot((int)c);
+        if (s >= 0) {
+            atomicAdd(&acc[6 * s + 0], imp.x);
+            atomicAdd(&acc[6 * s + 1], imp.y);
+            atomicAdd(&acc[6 * s + 2], imp.z);
+            atomicAdd(&acc[6 * s + 3], imd.x);
+            atomicAdd(&acc[6 * s + 4], imd.y);
+            atomicAdd(&acc[6 * s + 5], imd.z);
+            atomicAdd(&count[s], 1);
+        } else {
+            atomicAdd(dv + 3 * c + 0, imp.x);
+            atomicAdd(dv + 3 * c + 1, imp.y);
+            atomicAdd(dv + 3 * c + 2, imp.z);
+            atomicAdd(dx + 3 * c + 0, imd.x);
+            atomicAdd(dx + 3 * c + 1, imd.y);
+            atomicAdd(dx + 3 * c + 2, imd.z);
+            atomicAdd(cnt + c, 1);
+        }
     }
     __device__ void drain(int tid, T *dv, T *dx, int *cnt) {
         for (int s = tid; s < kImpSlots; s += kThreads) {
@@ -612,44 +639,7 @@ template <typename T> struct ImpTable {
         }
     }
 };
-static_assert(kImpSlots == 1 << 9, "the slot hash takes 9 bits");
-
-template <typename T>
-__device__ __forceinline__ void scatter_impulse_lds(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
-                                                    ImpTable<T> &tab, T *dv, T *dx, int *cnt) {
-    T q[4];
-    int side[4], idx[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        type_corner(type, k, side[k], idx[k]);
-        q[k] = masses[3 * (side[k] ? y : x) + idx[k]];
-    }
-    const ImpulseParams<T> ip = collision_impulse(m, type, root, q);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        V3<T> imp, imd;
-        corner_impulse(ip, type, k, m.dv[k], imp, imd);
-        const int64_t c = 3 * (side[k] ? y : x) + idx[k];
-        const int s = tab.slot((int)c);
-        if (s >= 0) {
-            atomicAdd(&tab.acc[6 * s + 0], imp.x);
-            atomicAdd(&tab.acc[6 * s + 1], imp.y);
-            atomicAdd(&tab.acc[6 * s + 2], imp.z);
-            atomicAdd(&tab.acc[6 * s + 3], imd.x);
-            atomicAdd(&tab.acc[6 * s + 4], imd.y);
-            atomicAdd(&tab.acc[6 * s + 5], imd.z);
-            atomicAdd(&tab.count[s], 1);
-        } else {
-            atomicAdd(dv + 3 * c + 0, imp.x);
-            atomicAdd(dv + 3 * c + 1, imp.y);
-            atomicAdd(dv + 3 * c + 2, imp.z);
-            atomicAdd(dx + 3 * c + 0, imd.x);
-            atomicAdd(dx + 3 * c + 1, imd.y);
-            atomicAdd(dx + 3 * c + 2, imd.z);
-            atomicAdd(cnt + c, 1);
-        }
-    }
-}
+static_assert(kImpSlots == 1 << kImpBits, "the slot count is a power of two");
 
 // Hides a pointer's provenance from the optimiser so that loads through it are not
 // merged with earlier loads of the same addresses.
@@ -660,38 +650,58 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 
 // One tile of kThreads items runs in four block phases, each with one record per thread,
 // so that lanes of a wave follow the same branch:
-//   A  item -> tetrahedron -> coefficients -> cubic_prepare (branch, f, g, h); records are
-//      binned by branch in LDS;
-//   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list;
+//   A  item -> tetrahedron -> coefficients -> cubic_prepare; the record (item and the
+//      doubles its branch's root formulas read) is written binned by branch to LDS;
+//   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list
+//      (record index, root);
 //   C  filterCollisions per root record (the tetrahedron is gathered again);
-//   D  (impulse mode) accepted collisions wait in an LDS stage; once a block's worth has
-//      gathered, they are scattered through an LDS hash table.  The scatter stays in this
-//      kernel: run as a kernel of its own over per-workgroup collision lists it took 14 ms
-//      per pass on the 130k cloth against ~6 here, where other workgroups' fp64 work hides
-//      its atomic latency (and that more than ate the 5-wave occupancy the CCD phases get
-//      without the scatter's registers).
-// The launch bound asks for five resident blocks per CU, which the collect mode reaches
-// (<= 102 VGPRs, 29 KB of LDS); the impulse mode stays at three (157 VGPRs, 45 KB of LDS).
-struct alignas(16) SolveRec {
+//   D  (impulse mode) accepted collisions wait in an LDS stage; kFlush at a time they are
+//      scattered through an LDS hash table that reuses the record and root-list memory:
+//      one thread per collision computes its impulse parameters (gathering the
+//      tetrahedron once for the weights and once for the rest, so that this step needs
+//      no more registers than the CCD phases), then one thread per corner adds to the
+//      table.  The scatter stays in this kernel: run as a kernel of its own over
+//      per-workgroup collision lists it took 14 ms per pass on the 130k cloth against ~6
+//      here, where other workgroups' fp64 work hides its atomic latency.
+// Each thread's item of the next tile is loaded while this tile is solved.  Occupancy: the
+// collect mode fits five blocks per CU (<= 96 VGPRs, 18 KB of LDS), the impulse mode four
+// (128 VGPRs, 31 KB).  At five blocks the impulse mode spills about a hundred registers and
+// ran 40% slower; at three (45 KB of LDS, the scatter inlined) 13% slower.
+struct SolveRec {
     int64_t item;
-    double a, b, c, d, f, g, h;
+    double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
 };
 constexpr int kRootCap = 3 * kThreads;
 
+__device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
+    if (code == CUBIC_QUADRATIC) { r.v[0] = st.b; r.v[1] = st.c; r.v[2] = st.d; r.v[3] = 0; }
+    else if (code == CUBIC_TRIPLE) { r.v[0] = st.a; r.v[1] = st.d; r.v[2] = 0; r.v[3] = 0; }
+    else { r.v[0] = st.a; r.v[1] = st.b; r.v[2] = st.g; r.v[3] = st.h; }
+}
+
+__device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
+    CubicState st{0, 0, 0, 0, 0, 0, 0};
+    if (code == CUBIC_QUADRATIC) { st.b = r.v[0]; st.c = r.v[1]; st.d = r.v[2]; }
+    else if (code == CUBIC_TRIPLE) { st.a = r.v[0]; st.d = r.v[1]; }
+    else { st.a = r.v[0]; st.b = r.v[1]; st.g = r.v[2]; st.h = r.v[3]; }
+    return st;
+}
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ t0_all, const T *__restrict__ t1_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   DevCounters *__restrict__ ctr_all) {
-    // s_rec (phases A-B) and the root list (phases B-C) share their LDS with the impulse
-    // table of phase D
-    __shared__ __align__(16) unsigned char s_raw[sizeof(SolveRec) * kThreads + (sizeof(int64_t) + sizeof(T)) * kRootCap];
+    // the records (phases A-C) and the root list (phases B-C) share their LDS with the
+    // impulse table of phase D
+    constexpr size_t kSolveBytes = sizeof(SolveRec) * kThreads + (sizeof(T) + sizeof(unsigned short)) * kRootCap;
+    constexpr size_t kScatterBytes = ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush;
+    __shared__ __align__(16) unsigned char s_raw[kSolveBytes > kScatterBytes ? kSolveBytes : kScatterBytes];
     SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
-    int64_t *s_ritem = reinterpret_cast<int64_t *>(s_raw + sizeof(SolveRec) * kThreads);
-    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kThreads + sizeof(int64_t) * kRootCap);
-    __shared__ unsigned short s_perm[kThreads];
+    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kThreads);
+    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * kThreads + sizeof(T) * kRootCap);
     __shared__ int s_cnt[5];
     __shared__ int s_nroot;
     __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
@@ -715,19 +725,52 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
     // Staged collisions are scattered kThreads at a time: their corner impulses are first
     // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
     // dx / cnt with one set of global atomics.
-    static_assert(ImpTable<T>::bytes() <= sizeof(s_raw), "the impulse table fits the phase A-C buffers");
+    static_assert(ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush <= sizeof(s_raw),
+                  "the impulse table and parameters fit the phase A-C buffers");
+    static_assert(kFlush <= kThreads && (4 * kFlush) % kThreads == 0, "whole rounds of corners");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
+    ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + ImpTable<T>::bytes());
+    // Scatters staged collisions [0, f) in chunks of kFlush: one thread per collision
+    // computes its impulse parameters, then one thread per tetrahedron corner adds that
+    // corner's share to the LDS table (the corner step holds few registers, so the
+    // collision step alone bounds this phase's register use).
     auto flush = [&](int f) {
-        for (int base = 0; base < f; base += kThreads) {
+        for (int base = 0; base < f; base += kFlush) {
+            const int nf = f - base < kFlush ? f - base : kFlush;
             tab.clear(tid);
-            __syncthreads();
-            const int j = base + tid;
-            if (j < f) {
+            if (tid < nf) {
                 int64_t x, y;
                 int type;
-                unpack_item(s_item[j], x, y, type);
-                const MTet<T> mt = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
-                scatter_impulse_lds(mt, type, s_root[j], x, y, masses, tab, dv, dx, cnt);
+                unpack_item(s_item[base + tid], x, y, type);
+                const T root = s_root[base + tid];
+                // the tetrahedron is gathered once per step (from L2) rather than held
+                // across both: that keeps this phase within the CCD phases' registers
+                const ImpulseWeights<T> w = impulse_weights(gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type), type, root);
+                T q[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    int side, idx;
+                    type_corner(type, k, side, idx);
+                    q[k] = masses[3 * (side ? y : x) + idx];
+                }
+                s_imp[tid] = impulse_params(gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type), type, root, w, q);
+            }
+            __syncthreads();
+            for (int t = tid; t < 4 * kFlush; t += kThreads) {
+                const int k = t / kFlush, jc = t % kFlush;
+                if (jc < nf) {
+                    int64_t x, y;
+                    int type;
+                    unpack_item(s_item[base + jc], x, y, type);
+                    int side, idx;
+                    type_corner(type, k, side, idx);
+                    const int64_t c = 3 * (side ? y : x) + idx;
+                    const V3<T> v = ld3(opaque_ptr(t0) + 3 * c);
+                    const V3<T> dvk = ld3(opaque_ptr(t1) + 3 * c) - v;
+                    V3<T> imp, imd;
+                    corner_impulse(s_imp[jc], type, k, dvk, imp, imd);
+                    tab.add(c, imp, imd, dv, dx, cnt);
+                }
             }
             __syncthreads();
             tab.drain(tid, dv, dx, cnt);
@@ -735,15 +778,22 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
         }
     };
 
+    // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
+    // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
     const int64_t ntiles = (M + kThreads - 1) / kThreads;
-    CCC_FOR_TILES(tile, ntiles) {
+    const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
+    const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
+    auto item_of = [&](int64_t k) -> int64_t {
+        const int64_t i = (tbase + k) * kThreads + tid;
+        return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
+    };
+    int64_t next_item = item_of(blockIdx.x / kXcds);
+    for (int64_t tk = blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
         // ---- A: coefficients and branch of one item per thread
-        const int64_t i = tile * kThreads + tid;
+        const int64_t it = next_item;
         int code = CUBIC_NONE;
         CubicState st;
-        int64_t it = 0;
-        if (i < M) {
-            it = items[i];
+        if (it >= 0) {
             int64_t x, y;
             int type;
             unpack_item(it, x, y, type);
@@ -764,18 +814,17 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
         const int tot = o4 + s_cnt[CUBIC_CARDANO];
         if (code != CUBIC_NONE) {
             const int p = (code == CUBIC_QUADRATIC ? 0 : code == CUBIC_TRIPLE ? o2 : code == CUBIC_TRIG ? o3 : o4) + pos;
-            s_perm[p] = (unsigned short)tid;
             SolveRec r;
-            r.item = it; r.a = st.a; r.b = st.b; r.c = st.c; r.d = st.d; r.f = st.f; r.g = st.g; r.h = st.h;
-            s_rec[tid] = r;
+            r.item = it;
+            rec_store(code, st, r);
+            s_rec[p] = r;
         }
         __syncthreads();
+        next_item = item_of(tk + tstep);
         // ---- B: finish the cubic on branch-binned records; roots in [0, 1] to the root list
         if (tid < tot) {
             const int c = tid < o2 ? CUBIC_QUADRATIC : tid < o3 ? CUBIC_TRIPLE : tid < o4 ? CUBIC_TRIG : CUBIC_CARDANO;
-            const SolveRec r = s_rec[s_perm[tid]];
-            CubicState rs;
-            rs.a = r.a; rs.b = r.b; rs.c = r.c; rs.d = r.d; rs.f = r.f; rs.g = r.g; rs.h = r.h;
+            const CubicState rs = rec_load(c, s_rec[tid]);
             T sl[3];
             cubic_finish(c, rs, sl);
 #pragma unroll
@@ -783,7 +832,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
                 if (sl[k] >= 0 && sl[k] <= 1) {
                     ++nroots;
                     const int p = atomicAdd(&s_nroot, 1);
-                    s_ritem[p] = r.item;
+                    s_rrec[p] = (unsigned short)tid;
                     s_rroot[p] = sl[k];
                 }
             }
@@ -797,7 +846,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
             int64_t rit = 0;
             T root = 0;
             if (j < nr) {
-                rit = s_ritem[j];
+                rit = s_rec[s_rrec[j]].item;
                 root = s_rroot[j];
                 int64_t x, y;
                 int type;
@@ -821,15 +870,21 @@ __global__ __launch_bounds__(kThreads, 5) void k_ccd(const int64_t *__restrict__
                 s_root[p] = root;
             }
         }
-        // ---- D: scatter staged impulses once a block's worth has gathered
+        // ---- D: scatter the staged impulses in whole chunks; the rest moves to the front
         if (MODE == MODE_IMPULSE) {
             __syncthreads();
             const int f = s_fill;
-            if (f >= kThreads) {
-                flush(f);
-                if (tid == 0) ncoll += (unsigned long long)f;
+            if (f >= kFlush) {
+                const int done = f - f % kFlush;
+                flush(done);
+                if (tid == 0) ncoll += (unsigned long long)done;
+                // [done, f) and [0, f - done) do not overlap: f - done < kFlush <= done
+                if (tid < f - done) {
+                    s_item[tid] = s_item[done + tid];
+                    s_root[tid] = s_root[done + tid];
+                }
                 __syncthreads();
-                if (tid == 0) s_fill = 0;
+                if (tid == 0) s_fill = f - done;
             }
         }
         __syncthreads();
