@@ -21,7 +21,11 @@ namespace ccc {
 
 constexpr int kThreads = 256;
 constexpr int kTravThreads = 256;
-constexpr int kStackDepth = 32;   // pending right children per lane (LDS)
+// Pending right children per lane (LDS).  The short stack covers the trees built here up
+// to 2^18 leaves below the frontier; a traversal that overflows it runs again with the
+// deep one.  Every pushed entry is a level of the tree below the start point.
+constexpr int kStackShort = 16;
+constexpr int kStackDeep = 32;
 constexpr int kStage = 256;       // staged pairs per wave before a flush
 constexpr int kFrontierDepth = 3; // queries start from the subtrees this deep below the root
 constexpr int kFrontier = 1 << kFrontierDepth;
@@ -334,7 +338,7 @@ __global__ void k_frontier(const T *__restrict__ bb_all, const int64_t *__restri
 // :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
 // reported after the reference's exact test checkOverlap(query box, leaf box, threshold) on
 // the boxes stored in `bboxes`; the pair is (min id, max id).
-template <typename T, bool PARTIAL>
+template <typename T, bool PARTIAL, int DEPTH>
 __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
                                                            const TNode *__restrict__ nodes_all,
                                                            const LeafRec *__restrict__ recs_all, int64_t n, T thr,
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
                                                            DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all) {
-    __shared__ int stack[kStackDepth][kTravThreads];
+    __shared__ int stack[DEPTH][kTravThreads];
     __shared__ int2 stage[kTravThreads / 64][kStage + 128];
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
@@ -451,7 +455,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             } else {
                 node = tl ? nd.d.x : nd.d.y;
                 if (tl && tr) {
-                    if (sp >= kStackDepth) { atomicOr(&ctr->flags, DEV_STACK); break; }
+                    if (sp >= DEPTH) { atomicOr(&ctr->flags, DEV_STACK); break; }
                     stack[sp++][tid] = nd.d.y;
                 }
             }
@@ -1214,16 +1218,22 @@ static ccc_status run_query(const Query &q) {
     int2 *pairs = nullptr;
     // one block per (start point, query block); query blocks rounded to a multiple of kXcds
     const unsigned gT = kFrontier * (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
+    bool deep = false;   // the short stack overflowed once: run with the deep one
     for (int attempt = 0; attempt < 2; ++attempt) {
         pairs = ws.get_as<int2>(B * capK);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         tm.begin("traverse");
-        if (q.to_check)
-            hipLaunchKernelGGL((k_traverse<T, true>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier);
-        else
-            hipLaunchKernelGGL((k_traverse<T, false>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree,
-                               nodes, recs, n, (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier);
+#define CCC_TRAVERSE(P, D)                                                                                   \
+    hipLaunchKernelGGL((k_traverse<T, P, D>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree, nodes, recs, n, \
+                       (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier)
+        if (q.to_check) {
+            if (deep) CCC_TRAVERSE(true, kStackDeep);
+            else CCC_TRAVERSE(true, kStackShort);
+        } else {
+            if (deep) CCC_TRAVERSE(false, kStackDeep);
+            else CCC_TRAVERSE(false, kStackShort);
+        }
+#undef CCC_TRAVERSE
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
@@ -1235,7 +1245,18 @@ static ccc_status run_query(const Query &q) {
             kmax = std::max(kmax, h[b].cand);
         }
         if (flags & DEV_BAD_TREE) { set_error("tree tensor is not a valid BVH over n leaves"); return CCC_ERR_BAD_TREE; }
-        if (flags & DEV_STACK) { set_error("BVH deeper than the traversal stack"); return CCC_ERR_STACK; }
+        if (flags & DEV_STACK) {
+            if (deep) { set_error("BVH deeper than the traversal stack"); return CCC_ERR_STACK; }
+            // again with the deep stack (not a capacity attempt)
+            deep = true;
+            --attempt;
+            for (int64_t b = 0; b < B; ++b) {
+                h[b].cand = 0;
+                h[b].flags &= ~DEV_STACK;
+            }
+            CCC_HIP_TRY(hipMemcpyAsync(ctr, h.data(), sizeof(DevCounters) * B, hipMemcpyHostToDevice, s));
+            continue;
+        }
         if (kmax <= (unsigned long long)capK) break;
         warn_cand = true;
         if (q.flags & CCC_FLAG_FIXED_CAPACITY) break;

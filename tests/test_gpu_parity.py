@@ -426,6 +426,75 @@ def test_unordered_tree_takes_the_climb(ccc, oracle):
     _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
 
 
+def _left_spine_tree(t0, t1, depth):
+    """A valid wire tree whose left spine is `depth` internal nodes long, each with a
+    two-leaf subtree as its right child (the bottom one has two): descending the spine
+    keeps every right subtree pending, so a traversal's stack grows with the depth.
+    Leaves (n = 2 depth + 2) are numbered in depth-first order; boxes are refit."""
+    n = 2 * depth + 2
+    assert t0.shape[0] == n
+    rows = 2 * n - 1
+    tree = np.full((rows, 4), -1, np.int64)
+    spine = list(range(depth))                   # S_i = row i, S_0 the root
+    pair = [depth + i for i in range(depth + 1)]  # P_i = row depth + i
+    for i in range(depth - 1):
+        tree[spine[i], 1], tree[spine[i + 1], 0] = spine[i + 1], spine[i]
+    tree[spine[depth - 1], 1], tree[pair[depth], 0] = pair[depth], spine[depth - 1]
+    for i in range(depth):
+        tree[spine[i], 2], tree[pair[i], 0] = pair[i], spine[i]
+    k = 0
+    for i in range(depth, -1, -1):               # depth-first leaf order: P_depth first
+        for c in (1, 2):
+            leaf = n - 1 + k
+            tree[pair[i], c], tree[leaf, 0], tree[leaf, 3] = leaf, pair[i], k
+            k += 1
+    bb = np.zeros((rows, 6))
+    for k in range(n):
+        v = np.concatenate([t0[k], t1[k]])
+        bb[n - 1 + k, :3], bb[n - 1 + k, 3:] = v.min(0), v.max(0)
+    for i in list(reversed(pair)) + list(reversed(spine)):   # children before parents
+        l, r = tree[i, 1], tree[i, 2]
+        bb[i, :3] = np.minimum(bb[l, :3], bb[r, :3])
+        bb[i, 3:] = np.maximum(bb[l, 3:], bb[r, 3:])
+    return bb, tree
+
+
+def _cluster(n, seed):
+    """n small moving triangles whose boxes all contain the point c: every pair overlaps."""
+    rng = np.random.default_rng(seed)
+    c = np.full(3, 0.05)
+    t0 = np.empty((n, 3, 3))
+    t0[:, 0] = c - rng.uniform(0.01, 0.05, (n, 3))
+    t0[:, 1] = c + rng.uniform(0.01, 0.05, (n, 3))
+    t0[:, 2] = c + rng.uniform(-0.05, 0.05, (n, 3))
+    t1 = t0 + rng.normal(0.0, 0.01, (n, 3, 3))
+    return t0, t1
+
+
+def test_deep_tree_reruns_with_the_deep_stack(ccc, oracle):
+    """A spine 24 levels deep overflows the short traversal stack below the frontier; the
+    traversal runs again with the deep stack and finds every pair the oracle finds."""
+    t0, t1 = _cluster(50, seed=21)
+    bb, tree = _left_spine_tree(t0, t1, 24)
+    pairs, total = oracle.traverse(bb, tree, 0.0)
+    assert total == 50 * 49 // 2
+    np.testing.assert_array_equal(sorted_rows(pairs), sorted_rows(oracle.bruteforce_pairs(bb, tree, 0.0)))
+    coll, roots, counts = oracle.ccd(t0, t1, pairs)
+    gc, gr = ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
+    assert ccc.last_counts()[0][0] == counts["candidates"] == total
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+
+
+def test_tree_deeper_than_the_deep_stack_raises(ccc, oracle):
+    """Beyond the deep stack (a 40-level spine: 37 pending subtrees below the frontier) the
+    call fails with an error instead of returning a partial pair list."""
+    t0, t1 = _cluster(82, seed=22)
+    bb, tree = _left_spine_tree(t0, t1, 40)
+    assert oracle.traverse(bb, tree, 0.0)[1] == 82 * 81 // 2     # the oracle's stack is deeper
+    with pytest.raises(RuntimeError, match="deeper than the traversal stack"):
+        ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
+
+
 def test_bad_tree_raises(ccc, oracle):
     t0, t1, _ = cases.random_soup(50, seed=1)
     rb, rt, _, _ = oracle.bvh(t0, t1)
