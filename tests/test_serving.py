@@ -85,3 +85,31 @@ def test_results_same_alone_and_in_the_mix(ccc):
         diff = float((r.result - alone).abs().max())
         assert diff <= 1e-9 * corr + 1e-15, (r.sim, diff, corr)
         assert r.done >= r.start >= r.arrival
+
+
+@pytest.mark.gpu
+def test_impulse_step_matches_the_oracle_step(ccc, oracle):
+    """The caller step on the GPU (bvh_motion + compute_impulses + mean-correction vertex
+    update, serving.impulse_step) against the same pass composed on the CPU from the oracle
+    and scenes.apply_corrections, for three passes in a row, each from the GPU's previous
+    positions: positions agree to 1e-9 of the pass's correction.  (Chained passes are not
+    compared end to end: the atomic summation order leaves 1e-17 differences in the
+    positions, and the next pass's borderline root and filter decisions amplify them.)"""
+    import torch
+    from cccollisions_amd import scenes
+    sc = scenes.folded_cloth(10, seed=7)
+    t0 = sc.x0[sc.faces]
+    m = sc.corner_masses()
+    x_prev = sc.x1.copy()
+    for k in range(3):
+        sim = serving.Simulation(torch, sc.faces, sc.x0, x_prev, m)
+        got = serving.impulse_step(ccc, torch, [sim], 1, 512)[0].cpu().numpy()
+        bb, tree = oracle.bvh(t0, x_prev[sc.faces])[:2]
+        _, dx, cnt, _, _, counts = oracle.compute_impulses(bb, tree, t0, x_prev[sc.faces], m)
+        assert counts["collisions"] > 0
+        ref = scenes.apply_corrections(sc.faces, x_prev, dx, cnt)
+        corr = float(np.abs(ref - x_prev).max())
+        assert corr > 0
+        diff = float(np.abs(got - ref).max())
+        assert diff <= 1e-9 * corr + 1e-15, (k, diff, corr)
+        x_prev = got
