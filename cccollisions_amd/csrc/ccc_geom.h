@@ -197,6 +197,31 @@ __device__ __forceinline__ MTet<T> gather_mtet(const T *t0, const T *t1, int64_t
     return m;
 }
 
+// Vertex records of the narrow phase: for each triangle corner its position at the start
+// and at the end of the step, [x0 y0 z0 x1 y1 z1] (48 B for double, 16-byte aligned), so
+// that one corner is one contiguous read (three 16-byte loads) instead of two 24-byte reads
+// from the t0 and t1 soups.
+template <typename T> struct alignas(2 * sizeof(T)) VRec { T a[6]; };
+
+template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64_t corner, V3<T> &v, V3<T> &dv) {
+    const VRec<T> r = reinterpret_cast<const VRec<T> *>(tv)[corner];
+    v = mk(r.a[0], r.a[1], r.a[2]);
+    dv = mk(r.a[3], r.a[4], r.a[5]) - v;
+}
+
+// gather_mtet from vertex records (same values, so the same tetrahedron bit for bit)
+template <typename T>
+__device__ __forceinline__ MTet<T> gather_mtet_rec(const T *tv, int64_t x, int64_t y, int type) {
+    MTet<T> m;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int side, idx;
+        type_corner(type, k, side, idx);
+        ld_vrec(tv, 3 * (side ? y : x) + idx, m.v[k], m.dv[k]);
+    }
+    return m;
+}
+
 // figures.cu:122-293 coefficients c3..c0 of det[(v_i - v3) + t (dv_i - dv3)], i = 0..2,
 // accumulated in the reference's term order; `coef += p*q*r` is fma(p*q, r, coef).
 template <typename T> __device__ __forceinline__ void cubic_coeffs(const MTet<T> &t, T *c) {

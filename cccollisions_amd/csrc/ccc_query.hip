@@ -536,6 +536,20 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
 //            with atomics into dv / dx / cnt (collisions_impulses.cu:1312-1366).
 // The tetrahedron is gathered again for the filter so that it is not live across the
 // cubic solver (fewer VGPRs, more waves per SIMD).
+// Vertex records [x0 y0 z0 x1 y1 z1] per triangle corner from the t0 / t1 soups.
+template <typename T>
+__global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restrict__ t1_all, int64_t corners,
+                                 T *__restrict__ tv_all) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // corner of batch element blockIdx.y
+    if (i >= corners) return;
+    const int64_t b = blockIdx.y;
+    const T *a = t0_all + (b * corners + i) * 3, *c = t1_all + (b * corners + i) * 3;
+    VRec<T> r;
+    r.a[0] = a[0]; r.a[1] = a[1]; r.a[2] = a[2];
+    r.a[3] = c[0]; r.a[4] = c[1]; r.a[5] = c[2];
+    reinterpret_cast<VRec<T> *>(tv_all)[b * corners + i] = r;
+}
+
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
 // tiles and a tile adds at most 3 * kThreads.
@@ -693,7 +707,7 @@ __device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
-                                                  const T *__restrict__ t0_all, const T *__restrict__ t1_all,
+                                                  const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
@@ -716,8 +730,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
     const unsigned long long mc = ctr->items;
     const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
     const int64_t *items = items_all + (int64_t)b * capM;
-    const T *t0 = t0_all + (int64_t)b * n * 9;
-    const T *t1 = t1_all + (int64_t)b * n * 9;
+    const T *tv = tv_all + (int64_t)b * n * 18;
     const T *masses = MODE == MODE_IMPULSE ? masses_all + (int64_t)b * n * 3 : nullptr;
     T *dv = MODE == MODE_IMPULSE ? dv_all + (int64_t)b * n * 9 : nullptr;
     T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
@@ -749,7 +762,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                 const T root = s_root[base + tid];
                 // the tetrahedron is gathered once per step (from L2) rather than held
                 // across both: that keeps this phase within the CCD phases' registers
-                const ImpulseWeights<T> w = impulse_weights(gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type), type, root);
+                const ImpulseWeights<T> w = impulse_weights(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root);
                 T q[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -757,7 +770,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                     type_corner(type, k, side, idx);
                     q[k] = masses[3 * (side ? y : x) + idx];
                 }
-                s_imp[tid] = impulse_params(gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type), type, root, w, q);
+                s_imp[tid] = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
             }
             __syncthreads();
             for (int t = tid; t < 4 * kFlush; t += kThreads) {
@@ -769,8 +782,8 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                     int side, idx;
                     type_corner(type, k, side, idx);
                     const int64_t c = 3 * (side ? y : x) + idx;
-                    const V3<T> v = ld3(opaque_ptr(t0) + 3 * c);
-                    const V3<T> dvk = ld3(opaque_ptr(t1) + 3 * c) - v;
+                    V3<T> v, dvk;
+                    ld_vrec(opaque_ptr(tv), c, v, dvk);
                     V3<T> imp, imd;
                     corner_impulse(s_imp[jc], type, k, dvk, imp, imd);
                     tab.add(c, imp, imd, dv, dx, cnt);
@@ -803,7 +816,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
             unpack_item(it, x, y, type);
             T cf[4];
             {
-                const MTet<T> m = gather_mtet(t0, t1, x, y, type);
+                const MTet<T> m = gather_mtet_rec(tv, x, y, type);
                 cubic_coeffs(m, cf);
             }
             code = cubic_prepare(cf[0], cf[1], cf[2], cf[3], st);
@@ -855,7 +868,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                 int64_t x, y;
                 int type;
                 unpack_item(rit, x, y, type);
-                const MTet<T> m = gather_mtet(opaque_ptr(t0), opaque_ptr(t1), x, y, type);
+                const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
                 ok = filter_collision(m, type, root);
             }
             if (MODE == MODE_COLLECT) {
@@ -1296,9 +1309,16 @@ static ccc_status run_query(const Query &q) {
         const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, kPairChunk), 1);
         const int64_t capM = 15 * chunk;
         int64_t *items = ws.get_as<int64_t>(B * capM);
+        const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
+        T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        if (ccd) {
+            hipLaunchKernelGGL((k_vertex_records<T>), dim3(grid_for(3 * n, 256), B), dim3(256), 0, s, (const T *)q.t0,
+                               (const T *)q.t1, 3 * n, tv);
+            CCC_LAUNCH_CHECK("k_vertex_records");
+        }
         auto narrow_phase = [&]() -> ccc_status {
             for (int64_t c0 = 0; c0 < kmax; c0 += chunk) {
                 const int64_t c1 = c0 + chunk;
@@ -1316,12 +1336,12 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
                                        q.out_rows, q.out_cap, ctr);
                 else if (q.narrow == Narrow::CCD_COLLECT)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                                       (const T *)q.t1, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)tv,
+                                       (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
                                        (T *)nullptr, (T *)nullptr, (int *)nullptr, ctr);
                 else
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0,
-                                       (const T *)q.t1, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)tv,
+                                       (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
                                        (T *)q.dv, (T *)q.dx, q.cnt, ctr);
                 tm.end();
                 CCC_LAUNCH_CHECK("narrow phase");

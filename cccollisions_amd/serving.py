@@ -37,14 +37,15 @@ class Request:
 def footprint_bytes(num_triangles: int, max_candidates: int, itemsize: int = 8) -> int:
     """HBM a one-mesh step needs beyond the resident state: the two soups, the BVH and its
     traversal nodes and leaf records, the candidate list at capacity, one chunk of
-    sub-primitive items and the impulse outputs."""
+    sub-primitive items, the narrow phase's vertex records and the impulse outputs."""
     n = num_triangles
     soups = 2 * n * 9 * itemsize
     bvh = (2 * n - 1) * (6 * itemsize + 4 * 8) + (n - 1) * 64 + n * 32
     pairs = max_candidates * n * 8
     items = 15 * min(max_candidates * n, 1 << 25) * 8
     outs = n * (18 * itemsize + 12)
-    return soups + bvh + pairs + items + outs
+    records = n * 18 * itemsize        # per-corner [t0 | t1] vertex records of the narrow phase
+    return soups + bvh + pairs + items + outs + records
 
 
 class Scheduler:
