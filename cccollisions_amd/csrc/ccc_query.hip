@@ -1154,6 +1154,7 @@ constexpr int64_t kPairChunk = (int64_t)1 << 25;
 
 static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B) {
     int64_t want = (items + kThreads - 1) / kThreads;
+    if (want < 1) want = 1;                        // no items: a grid of idle blocks, never 0
     int64_t limit = (int64_t)ctx->num_cus * 16 / (B > 0 ? B : 1);
     if (limit < kXcds) limit = kXcds;
     if (want > limit) want = limit;

@@ -495,6 +495,45 @@ def test_tree_deeper_than_the_deep_stack_raises(ccc, oracle):
         ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
 
 
+def test_single_triangle_queries(ccc, oracle):
+    """N = 1 (the reference cannot build it): a one-leaf tree, no pairs, padded outputs and
+    zero impulses from every query."""
+    t0, t1, m = cases.random_soup(1, seed=3)
+    m = np.ones((1, 3))
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    np.testing.assert_array_equal(_np(tree)[0], rt)
+    gc, gr = ccc.find_collisions_continuous(bb, tree, _t(t0[None]), _t(t1[None]), 4, 4)
+    assert (_np(gc) == -1).all() and (_np(gr) == -1).all()
+    assert ccc.last_counts()[0][0] == 0
+    dv, dx, cnt = ccc.compute_impulses(bb, tree, _t(t0[None]), _t(t1[None]), _t(m[None]), 4, 4)
+    assert not _np(dv).any() and not _np(dx).any() and not _np(cnt).any()
+    assert (_np(ccc.find_collisions(bb, tree, _t(t0[None]), 0.0, 4)) == -1).all()
+    [close] = ccc.find_proximity(bb, tree, _t(t0[None]), 2e-3, 4)
+    assert (_np(close) == -1).all()
+
+
+def test_queries_without_candidates(ccc, oracle):
+    """Triangles far apart: no candidate pair, so every narrow phase runs over an empty list
+    (each launches a grid of idle blocks) and the outputs stay padded."""
+    t0, t1, m = cases.random_soup(5, seed=4)
+    t0 = t0 * 1e-3 + 10.0 * np.arange(5)[:, None, None]
+    t1 = t0 + 1e-4
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    assert oracle.traverse(*oracle.bvh(t0, t1)[:2], 0.0)[1] == 0
+    assert (_np(ccc.find_collisions(bb, tree, _t(t0[None]), 0.0, 4)) == -1).all()
+    gc, gr = ccc.find_collisions_continuous(bb, tree, _t(t0[None]), _t(t1[None]), 4, 4)
+    assert (_np(gc) == -1).all() and ccc.last_counts()[0][0] == 0
+    assert (_np(ccc.find_mincol_cand(bb, tree, _t(t0[None]), _t(t1[None]), 4)) == -1).all()
+    coll, coeff = ccc.find_triangle_edge_collisions(bb, tree, _t(t0[None]), 0.0, 4)
+    assert (_np(coll) == -1).all()
+    assert (_np(ccc.find_triangle_edge_candidates(bb, tree, _t(t0[None]), 0.0, 4)) == -1).all()
+    pairs, masks = ccc.find_triangle_edge_candidates2(bb, tree, _t(t0[None]), 0.0, 4)
+    assert (_np(pairs) == -1).all()
+    dv, dx, cnt = ccc.compute_impulses(bb, tree, _t(t0[None]), _t(t1[None]), _t(m[None]), 4, 4)
+    assert not _np(cnt).any()
+
+
 def test_bad_tree_raises(ccc, oracle):
     t0, t1, _ = cases.random_soup(50, seed=1)
     rb, rt, _, _ = oracle.bvh(t0, t1)
