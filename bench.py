@@ -52,7 +52,7 @@ def parse():
                          "median 20k triangles; 0: skip)")
     ap.add_argument("--serving-max-tris", type=int, default=200000)
     ap.add_argument("--serving-requests", type=int, default=64)
-    ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--serving-load", type=float, default=0.7,
                     help="offered load of the open-loop run as a fraction of the saturation rate")
     ap.add_argument("--main-only", action="store_true",

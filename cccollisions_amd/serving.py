@@ -151,7 +151,7 @@ def impulse_step(ccc, torch, sims: Sequence[Simulation], iters: int, max_candida
 class CollisionServer:
     """Worker slots over a Scheduler.  Each slot: one host thread, one HIP stream."""
 
-    def __init__(self, ccc, torch, sims: Sequence[Simulation], slots: int = 2, hbm_budget: Optional[int] = None,
+    def __init__(self, ccc, torch, sims: Sequence[Simulation], slots: int = 4, hbm_budget: Optional[int] = None,
                  iters: int = 10, max_candidates: int = 512):
         self.ccc, self.torch, self.sims = ccc, torch, list(sims)
         self.iters, self.max_candidates = iters, max_candidates

@@ -230,6 +230,40 @@ def test_fp32_continuous(ccc, oracle):
     np.testing.assert_array_equal(g, sorted_rows(coll))
 
 
+def test_fp32_queries(ccc, oracle):
+    """fp32 soups through the other queries: pair lists, collision rows and roots bit-exact
+    against the oracle's fp32 instantiation (the cubic is solved in double for both dtypes,
+    then rounded to float); impulses within 1e-5 relative (fp32 sums in another atomic
+    order: a few float ulps)."""
+    t0, t1, m = cases.random_soup(300, seed=7, dtype=np.float32)
+    m = m.astype(np.float32)
+    rb, rt, coll, roots, counts = _ccd_ref(oracle, t0, t1)
+    assert counts["collisions"] > 0
+    gc, gr = ccc.find_collisions_continuous(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), 64, 64)
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+    dv, dx, cnt = oracle.impulses(t0, t1, m, coll, roots)
+    g = ccc.compute_impulses(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), _t(m[None]), 64, 16)
+    np.testing.assert_array_equal(_np(g[2])[0], cnt)
+    for a, b in ((_np(g[0])[0], dv), (_np(g[1])[0], dx)):
+        fin = np.isfinite(b)
+        np.testing.assert_array_equal(np.isfinite(a), fin)
+        scale = float(np.abs(b[fin]).max())
+        np.testing.assert_allclose(a[fin], b[fin], rtol=1e-5, atol=1e-6 * scale)
+    ref = oracle.find_collisions(rb, rt, t0, 1e-3)
+    gs = ccc.find_collisions(_t(rb[None]), _t(rt[None]), _t(t0[None]), 1e-3, 64)
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(gs)[0])), sorted_rows(ref))
+    pairs, _ = oracle.traverse(rb, rt, 0.0)
+    gm = ccc.find_mincol_cand(_t(rb[None]), _t(rt[None]), _t(t0[None]), _t(t1[None]), 64)
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(gm)[0])), sorted_rows(oracle.mincol(t0, t1, pairs)))
+    tri = cases.proximity_fold(24, gap=1e-3).astype(np.float32)
+    pb, pt, _, _ = oracle.bvh(tri)
+    rows, pc = oracle.find_proximity(pb, pt, tri, 2e-3)
+    assert pc["collisions"] > 0
+    mc = int(np.ceil(max(pc["candidates"], pc["collisions"]) / len(tri))) + 1
+    gp = ccc.find_proximity(_t(pb[None]), _t(pt[None]), _t(tri[None]), 2e-3, mc)[0]
+    np.testing.assert_array_equal(sorted_rows(valid_rows(_np(gp)[0])), sorted_rows(rows))
+
+
 def test_partial(ccc, oracle):
     t0, t1, m = cases.random_soup(300, seed=9)
     rb, rt, _, _ = oracle.bvh(t0, t1)
