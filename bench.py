@@ -51,9 +51,9 @@ def parse():
                     help="also run the serving loop over this many resident folded cloths (lognormal sizes, "
                          "median 20k triangles; 0: skip)")
     ap.add_argument("--serving-max-tris", type=int, default=200000)
-    ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--serving-requests", type=int, default=128)
     ap.add_argument("--serving-slots", type=int, default=4)
-    ap.add_argument("--serving-load", type=float, default=0.7,
+    ap.add_argument("--serving-load", type=float, default=0.5,
                     help="offered load of the open-loop run as a fraction of the saturation rate")
     ap.add_argument("--main-only", action="store_true",
                     help="time only the 130k-triangle cloth (no garment, batched or serving section)")
