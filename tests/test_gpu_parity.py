@@ -116,6 +116,28 @@ def test_compute_impulses(ccc, oracle, name):
     _check_close(_np(g[1])[0], dx, "dx")
 
 
+def test_garment_full_size_first_pass(ccc, oracle):
+    """The garment-on-body scene at full size (23k-triangle body + 99k-triangle sleeve): one
+    impulse pass on the GPU against the oracle's multithreaded pass over the same soups:
+    counts equal, cnt exact, dv / dx within 1e-9 relative."""
+    from cccollisions_amd import scenes
+    sc = scenes.garment_on_body()
+    t0, t1 = sc.soups()
+    m = sc.corner_masses()
+    dv, dx, cnt, counts = oracle.impulse_pass(t0, t1, m, nthreads=16)
+    assert counts["collisions"] > 100000
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    mc = int(np.ceil(counts["candidates"] / len(t0))) + 1
+    g = ccc.compute_impulses(bb, tree, _t(t0[None]), _t(t1[None]), _t(m[None]), mc, 16)
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, V, R) == (counts["candidates"], counts["subprims"], counts["roots"], counts["collisions"])
+    np.testing.assert_array_equal(_np(g[2])[0], cnt)
+    for a, b, name in ((_np(g[0])[0], dv, "dv"), (_np(g[1])[0], dx, "dx")):
+        np.testing.assert_array_equal(np.isfinite(a), np.isfinite(b))
+        fin = np.isfinite(b)
+        _check_close(np.where(fin, a, 0.0), np.where(fin, b, 0.0), name)
+
+
 PACKED_MESHES = [
     lambda: cases.folded(8, seed=1),
     lambda: cases.random_soup(300, seed=8),
