@@ -70,9 +70,10 @@ __device__ __forceinline__ void unpack_item(int64_t it, int64_t &x, int64_t &y, 
 }
 
 // Block-wide reservation of `cnt` (< 16) consecutive slots per thread with one atomic.
-// Every thread of the block must call it.
+// Every thread of the block (of NT threads) must call it.
+template <int NT = kThreads>
 __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *counter, unsigned cnt) {
-    __shared__ unsigned long long s_wave[kThreads / 64];
+    __shared__ unsigned long long s_wave[NT / 64];
     __shared__ unsigned long long s_base;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
@@ -82,7 +83,7 @@ __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long tot = 0;
-        for (int q = 0; q < kThreads / 64; ++q) {
+        for (int q = 0; q < NT / 64; ++q) {
             unsigned long long v = s_wave[q];
             s_wave[q] = tot;
             tot += v;
@@ -551,10 +552,14 @@ __global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restri
 }
 
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
+// Threads (items) per k_ccd tile.  Two waves per block rather than four: the block phases
+// wait at barriers for their slowest wave, and 8 blocks of 2 waves per CU keep the SIMDs
+// busier than 4 of 4 (k_ccd 206 -> 193 ms per step on the 130k cloth).
+constexpr int kCcdThreads = 128;
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
-// tiles and a tile adds at most 3 * kThreads.
-constexpr int kFlush = kThreads / 4;
-constexpr int kImpStage = kFlush + 3 * kThreads;
+// tiles and a tile adds at most 3 * kCcdThreads.
+constexpr int kFlush = kCcdThreads / 4;
+constexpr int kImpStage = kFlush + 3 * kCcdThreads;
 
 template <typename T>
 __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T root, int64_t x, int64_t y, const T *masses,
@@ -604,7 +609,7 @@ template <typename T> struct ImpTable {
         return t;
     }
     __device__ void clear(int tid) {
-        for (int s = tid; s < kImpSlots; s += kThreads) {
+        for (int s = tid; s < kImpSlots; s += kCcdThreads) {
             key[s] = -1;
             count[s] = 0;
 #pragma unroll
@@ -644,7 +649,7 @@ template <typename T> struct ImpTable {
         }
     }
     __device__ void drain(int tid, T *dv, T *dx, int *cnt) {
-        for (int s = tid; s < kImpSlots; s += kThreads) {
+        for (int s = tid; s < kImpSlots; s += kCcdThreads) {
             const int k = key[s];
             if (k < 0) continue;
             atomicAdd(dv + 3 * (int64_t)k + 0, acc[6 * s + 0]);
@@ -666,7 +671,7 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
     return p;
 }
 
-// One tile of kThreads items runs in four block phases, each with one record per thread,
+// One tile of kCcdThreads items runs in four block phases, each with one record per thread,
 // so that lanes of a wave follow the same branch:
 //   A  item -> tetrahedron -> coefficients -> cubic_prepare; the record (item and the
 //      doubles its branch's root formulas read) is written binned by branch to LDS;
@@ -689,7 +694,7 @@ struct SolveRec {
     int64_t item;
     double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
 };
-constexpr int kRootCap = 3 * kThreads;
+constexpr int kRootCap = 3 * kCcdThreads;
 
 __device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
     if (code == CUBIC_QUADRATIC) { r.v[0] = st.b; r.v[1] = st.c; r.v[2] = st.d; r.v[3] = 0; }
@@ -706,7 +711,7 @@ __device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
@@ -714,12 +719,12 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                                                   DevCounters *__restrict__ ctr_all) {
     // the records (phases A-C) and the root list (phases B-C) share their LDS with the
     // impulse table of phase D
-    constexpr size_t kSolveBytes = sizeof(SolveRec) * kThreads + (sizeof(T) + sizeof(unsigned short)) * kRootCap;
+    constexpr size_t kSolveBytes = sizeof(SolveRec) * kCcdThreads + (sizeof(T) + sizeof(unsigned short)) * kRootCap;
     constexpr size_t kScatterBytes = ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush;
     __shared__ __align__(16) unsigned char s_raw[kSolveBytes > kScatterBytes ? kSolveBytes : kScatterBytes];
     SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
-    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kThreads);
-    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * kThreads + sizeof(T) * kRootCap);
+    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kCcdThreads);
+    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * kCcdThreads + sizeof(T) * kRootCap);
     __shared__ int s_cnt[5];
     __shared__ int s_nroot;
     __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
@@ -739,12 +744,12 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
     if (tid == 0) s_fill = 0;
     __syncthreads();
 
-    // Staged collisions are scattered kThreads at a time: their corner impulses are first
+    // Staged collisions are scattered kFlush at a time: their corner impulses are first
     // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
     // dx / cnt with one set of global atomics.
     static_assert(ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush <= sizeof(s_raw),
                   "the impulse table and parameters fit the phase A-C buffers");
-    static_assert(kFlush <= kThreads && (4 * kFlush) % kThreads == 0, "whole rounds of corners");
+    static_assert(kFlush <= kCcdThreads && (4 * kFlush) % kCcdThreads == 0, "whole rounds of corners");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
     ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + ImpTable<T>::bytes());
     // Scatters staged collisions [0, f) in chunks of kFlush: one thread per collision
@@ -773,7 +778,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                 s_imp[tid] = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
             }
             __syncthreads();
-            for (int t = tid; t < 4 * kFlush; t += kThreads) {
+            for (int t = tid; t < 4 * kFlush; t += kCcdThreads) {
                 const int k = t / kFlush, jc = t % kFlush;
                 if (jc < nf) {
                     int64_t x, y;
@@ -797,11 +802,11 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
     // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
-    const int64_t ntiles = (M + kThreads - 1) / kThreads;
+    const int64_t ntiles = (M + kCcdThreads - 1) / kCcdThreads;
     const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
     const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
     auto item_of = [&](int64_t k) -> int64_t {
-        const int64_t i = (tbase + k) * kThreads + tid;
+        const int64_t i = (tbase + k) * kCcdThreads + tid;
         return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
     };
     int64_t next_item = item_of(blockIdx.x / kXcds);
@@ -857,7 +862,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
         __syncthreads();
         // ---- C: filter one root record per thread
         const int nr = s_nroot;
-        for (int base = 0; base < nr; base += kThreads) {
+        for (int base = 0; base < nr; base += kCcdThreads) {
             const int j = base + tid;
             bool ok = false;
             int64_t rit = 0;
@@ -872,7 +877,7 @@ __global__ __launch_bounds__(kThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(
                 ok = filter_collision(m, type, root);
             }
             if (MODE == MODE_COLLECT) {
-                const unsigned long long slot = block_reserve(&ctr->coll, ok ? 1u : 0u);
+                const unsigned long long slot = block_reserve<kCcdThreads>(&ctr->coll, ok ? 1u : 0u);
                 if (ok && slot < (unsigned long long)out_cap) {
                     int64_t x, y;
                     int type;
@@ -1156,8 +1161,8 @@ constexpr int64_t kPairChunk = (int64_t)1 << 25;
 // then walking several tiles.  Measured on the 130k cloth (k_ccd per step): 4 blocks per CU
 // (all resident at once) 232 ms, 16 215 ms, 32 210, 128 207, 512 214 -- many short blocks
 // let the dispatcher balance the uneven tiles; past that the per-block start-up dominates.
-static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B) {
-    int64_t want = (items + kThreads - 1) / kThreads;
+static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B, int block = kThreads) {
+    int64_t want = (items + block - 1) / block;
     if (want < 1) want = 1;                        // no items: a grid of idle blocks, never 0
     int64_t limit = (int64_t)ctx->num_cus * 128 / (B > 0 ? B : 1);
     if (limit < kXcds) limit = kXcds;
@@ -1318,6 +1323,7 @@ static ccc_status run_query(const Query &q) {
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
+        const unsigned gCcd = persistent_grid(q.ctx, capM, B, kCcdThreads);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         if (ccd) {
             hipLaunchKernelGGL((k_vertex_records<T>), dim3(grid_for(3 * n, 256), B), dim3(256), 0, s, (const T *)q.t0,
@@ -1341,11 +1347,11 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
                                        q.out_rows, q.out_cap, ctr);
                 else if (q.narrow == Narrow::CCD_COLLECT)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)tv,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gCcd, B), dim3(kCcdThreads), 0, s, items, capM, (const T *)tv,
                                        (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
                                        (T *)nullptr, (T *)nullptr, (int *)nullptr, ctr);
                 else
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)tv,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gCcd, B), dim3(kCcdThreads), 0, s, items, capM, (const T *)tv,
                                        (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
                                        (T *)q.dv, (T *)q.dx, q.cnt, ctr);
                 tm.end();
