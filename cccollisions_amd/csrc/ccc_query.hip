@@ -20,7 +20,7 @@
 namespace ccc {
 
 constexpr int kThreads = 256;
-constexpr int kTravThreads = 256;
+constexpr int kTravThreads = 128;
 // Pending right children per lane (LDS).  The short stack covers the trees built here up
 // to 2^18 leaves below the frontier; a traversal that overflows it runs again with the
 // deep one.  Every pushed entry is a level of the tree below the start point.
