@@ -26,7 +26,7 @@ constexpr int kTravThreads = 128;
 // deep one.  Every pushed entry is a level of the tree below the start point.
 constexpr int kStackShort = 16;
 constexpr int kStackDeep = 32;
-constexpr int kStage = 256;       // staged pairs per wave before a flush
+constexpr int kStage = 512;       // staged pairs per wave before a flush
 constexpr int kFrontierDepth = 3; // queries start from the subtrees this deep below the root
 constexpr int kFrontier = 1 << kFrontierDepth;
 
