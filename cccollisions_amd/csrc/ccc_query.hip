@@ -349,6 +349,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all) {
     __shared__ int stack[DEPTH][kTravThreads];
     __shared__ int2 stage[kTravThreads / 64][kStage + 128];
+    __shared__ unsigned short stag[kTravThreads / 64][kStage + 128];   // lane << 10 | rank within the lane
+    __shared__ int loff[kTravThreads / 64][64];                         // a flush's offset of each lane
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
     const int64_t rows = 2 * n - 1;
@@ -372,22 +374,53 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     if (PARTIAL && active) active = to_check_all[(int64_t)b * n + p] != 0;
     const uint8_t *to_check = PARTIAL ? to_check_all + (int64_t)b * n : nullptr;
 
+    // Query setup (inactive lanes: no query; they stay in the loop below as idle lanes).
+    Box<T> q{};
+    int qid = 0, lim = INT_MAX;
+    float qx0 = 0, qy0 = 0, qz0 = 0, qx1 = 0, qy1 = 0, qz1 = 0;
+    bool alive = false;
     if (active) {
         const int64_t qrow = n - 1 + p;
-        const Box<T> q = ld_box(bb + 6 * qrow);
-        const int qid = recs[p].id;
-        const float qx0 = f32_down((double)q.mn.x), qy0 = f32_down((double)q.mn.y), qz0 = f32_down((double)q.mn.z);
-        const float qx1 = f32_up((double)q.mx.x), qy1 = f32_up((double)q.mx.y), qz1 = f32_up((double)q.mx.z);
+        q = ld_box(bb + 6 * qrow);
+        qid = recs[p].id;
+        qx0 = f32_down((double)q.mn.x); qy0 = f32_down((double)q.mn.y); qz0 = f32_down((double)q.mn.z);
+        qx1 = f32_up((double)q.mx.x); qy1 = f32_up((double)q.mx.y); qz1 = f32_up((double)q.mx.z);
         // packed meshes: leaves at positions >= lim belong to later meshes
-        const int lim = mesh_end ? mesh_end[p] : INT_MAX;
+        lim = mesh_end ? mesh_end[p] : INT_MAX;
         // the start point itself: box, rightmost and first-leaf tests as for a child
         const float *fb = fr.box[s];
-        bool enter = qx0 <= fb[3] && qx1 >= fb[0] && qy0 <= fb[4] && qy1 >= fb[1] && qz0 <= fb[5] && qz1 >= fb[2];
-        if (!PARTIAL) enter = enter && fr.rightmost[s] > (int)p;
-        enter = enter && fr.first[s] < lim;
-        int sp = 0, node = fr.ref[s];
-        while (enter) {
-            TNode nd;
+        alive = qx0 <= fb[3] && qx1 >= fb[0] && qy0 <= fb[4] && qy1 >= fb[1] && qz0 <= fb[5] && qz1 >= fb[2];
+        if (!PARTIAL) alive = alive && fr.rightmost[s] > (int)p;
+        alive = alive && fr.first[s] < lim;
+    }
+    const int lane = tid & 63;
+    int sp = 0, node = fr.ref[s];
+    int mine = 0;   // pairs this lane has staged since the last flush
+    // Writes the wave's nf staged pairs grouped by lane (lane order, each lane's pairs in the
+    // order found), so that the pair list keeps one query's pairs together.  Full wave.
+    auto flush_stage = [&](int nf) {
+        int incl = mine;    // inclusive scan of `mine` over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        loff[w][lane] = incl - mine;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&ctr->cand, (unsigned long long)nf);
+        base = readfirstlane_u64(base);
+        for (int j = lane; j < nf; j += 64) {
+            const unsigned t = stag[w][j];
+            const unsigned long long d = base + (unsigned long long)(loff[w][t >> 10] + (int)(t & 1023u));
+            if (d < (unsigned long long)cap) pairs[d] = stage[w][j];
+        }
+        mine = 0;
+    };
+    while (__any(alive)) {
+        int oid0 = -1, oid1 = -1;
+        bool ol = false, orr = false;
+        TNode nd;
+        if (alive) {
             if (node >= 0) {
                 nd = nodes[node];
             } else {
@@ -397,8 +430,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                 nd.c = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
                 nd.d = make_int4(node, 0, ~node, INT_MIN);
             }
-            bool ol = qx0 <= nd.a.w && qx1 >= nd.a.x && qy0 <= nd.b.x && qy1 >= nd.a.y && qz0 <= nd.b.y && qz1 >= nd.a.z;
-            bool orr = qx0 <= nd.c.y && qx1 >= nd.b.z && qy0 <= nd.c.z && qy1 >= nd.b.w && qz0 <= nd.c.w && qz1 >= nd.c.x;
+            ol = qx0 <= nd.a.w && qx1 >= nd.a.x && qy0 <= nd.b.x && qy1 >= nd.a.y && qz0 <= nd.b.y && qz1 >= nd.a.z;
+            orr = qx0 <= nd.c.y && qx1 >= nd.b.z && qy0 <= nd.c.z && qy1 >= nd.b.w && qz0 <= nd.c.w && qz1 >= nd.c.x;
             if (!PARTIAL) {
                 ol = ol && nd.d.z > (int)p;
                 orr = orr && nd.d.w > (int)p;
@@ -407,7 +440,6 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             // here cover contiguous position ranges); a node is only entered when its first
             // leaf is below lim, so the left child never needs the test
             orr = orr && nd.d.z + 1 < lim;
-            int oid0 = -1, oid1 = -1;
             if (ol && nd.d.x < 0) {
                 const LeafRec lr = recs[~nd.d.x];
                 if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~nd.d.x))), thr)) {
@@ -422,54 +454,46 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                     if (!PARTIAL || !to_check[oid] || qid < oid) oid1 = oid;
                 }
             }
-            // stage this iteration's pairs in the wave's LDS buffer
-            {
-                const unsigned c = (oid0 >= 0 ? 1u : 0u) + (oid1 >= 0 ? 1u : 0u);
-                const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u);
-                const unsigned below = lanes_below(b0) + 2u * lanes_below(b1);
-                const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1);
-                if (tot) {
-                    const int f = fill[w];
-                    int pos = f + (int)below;
-                    if (oid0 >= 0) stage[w][pos++] = make_int2(min(qid, oid0), max(qid, oid0));
-                    if (oid1 >= 0) stage[w][pos] = make_int2(min(qid, oid1), max(qid, oid1));
-                    const int nf = f + (int)tot;
-                    const unsigned long long act = __ballot(1);
-                    if (nf >= kStage) {
-                        unsigned long long base = 0;
-                        if (lanes_below(act) == 0) base = atomicAdd(&ctr->cand, (unsigned long long)nf);
-                        base = readfirstlane_u64(base);
-                        const int nact = __popcll(act);
-                        for (int j = (int)lanes_below(act); j < nf; j += nact)
-                            if (base + (unsigned long long)j < (unsigned long long)cap) pairs[base + j] = stage[w][j];
-                        if (lanes_below(act) == 0) fill[w] = 0;
-                    } else {
-                        if (lanes_below(act) == 0) fill[w] = nf;
-                    }
+        }
+        // stage this iteration's pairs in the wave's LDS buffer (tagged with lane and rank)
+        {
+            const unsigned c = (oid0 >= 0 ? 1u : 0u) + (oid1 >= 0 ? 1u : 0u);
+            const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u);
+            const unsigned below = lanes_below(b0) + 2u * lanes_below(b1);
+            const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1);
+            if (tot) {
+                const int f = fill[w];
+                int pos = f + (int)below;
+                const unsigned lane_tag = (unsigned)lane << 10;
+                if (oid0 >= 0) { stage[w][pos] = make_int2(min(qid, oid0), max(qid, oid0)); stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
+                if (oid1 >= 0) { stage[w][pos] = make_int2(min(qid, oid1), max(qid, oid1)); stag[w][pos] = (unsigned short)(lane_tag | mine++); }
+                const int nf = f + (int)tot;
+                if (nf >= kStage) {
+                    flush_stage(nf);
+                    if (lane == 0) fill[w] = 0;
+                } else {
+                    if (lane == 0) fill[w] = nf;
                 }
             }
+        }
+        if (alive) {
             const bool tl = ol && nd.d.x >= 0, tr = orr && nd.d.y >= 0;
             if (!tl && !tr) {
-                if (sp == 0) break;
-                --sp;
-                node = stack[sp][tid];
+                if (sp == 0) alive = false;
+                else node = stack[--sp][tid];
             } else {
                 node = tl ? nd.d.x : nd.d.y;
                 if (tl && tr) {
-                    if (sp >= DEPTH) { atomicOr(&ctr->flags, DEV_STACK); break; }
-                    stack[sp++][tid] = nd.d.y;
+                    if (sp >= DEPTH) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
+                    else stack[sp++][tid] = nd.d.y;
                 }
             }
         }
     }
-    // final flush of the wave's staged pairs (all lanes of the wave take part)
-    const int f = fill[w];
-    if (f > 0) {
-        unsigned long long base = 0;
-        if ((tid & 63) == 0) base = atomicAdd(&ctr->cand, (unsigned long long)f);
-        base = readfirstlane_u64(base);
-        for (int j = tid & 63; j < f; j += 64)
-            if (base + (unsigned long long)j < (unsigned long long)cap) pairs[base + j] = stage[w][j];
+    // final flush of the wave's staged pairs
+    {
+        const int f = fill[w];
+        if (f > 0) flush_stage(f);
     }
 }
 
