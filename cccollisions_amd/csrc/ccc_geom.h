@@ -359,10 +359,11 @@ template <typename T> __device__ __forceinline__ void cubic_finish(int code, con
         double j = pm_cbrt(i);
         double k = pm_acos(-(g / (2 * i)));
         double L = j * -1;
+        // cos(k / 3) appears twice in the formulas; evaluated once (the same value)
         double M = pm_cos(k / 3.0);
         double N = sqrt(3.0) * pm_sin(k / 3.0);
         double P = (b / (3.0 * a)) * -1;
-        double x1 = 2 * j * pm_cos(k / 3.0) - (b / (3.0 * a));
+        double x1 = 2 * j * M - (b / (3.0 * a));
         double x2 = L * (M + N) + P;
         double x3 = L * (M - N) + P;
         add_root(slots, x1, n);

@@ -21,13 +21,15 @@ namespace ccc {
 
 constexpr int kThreads = 256;
 constexpr int kTravThreads = 128;
-// Pending right children per lane (LDS).  The short stack covers the trees built here up
-// to 2^18 leaves below the frontier; a traversal that overflows it runs again with the
-// deep one.  Every pushed entry is a level of the tree below the start point.
+// Pending right children per lane (LDS).  Every pushed entry is a level of the tree below
+// the start point, so with start points kFrontierDepth = 7 levels down the short stack
+// covers the trees built here (index radix trees, ceil(log2 N) + 1 deep) up to 2^22
+// leaves; a traversal that overflows it runs again with the deep one (trees up to 39
+// levels deep).
 constexpr int kStackShort = 16;
 constexpr int kStackDeep = 32;
 constexpr int kStage = 512;       // staged pairs per wave before a flush
-constexpr int kFrontierDepth = 3; // queries start from the subtrees this deep below the root
+constexpr int kFrontierDepth = 7; // queries start from the subtrees this deep below the root
 constexpr int kFrontier = 1 << kFrontierDepth;
 
 // Workgroups are dispatched to the eight XCDs round-robin by workgroup ID, and each XCD has
@@ -276,62 +278,62 @@ template <typename T> __device__ __forceinline__ bool overlap_inward(const Box<T
            (q.mx.y >= ((T)r.mn[1] - thr)) && (q.mn.z <= ((T)r.mx[2] + thr)) && (q.mx.z >= ((T)r.mn[2] - thr));
 }
 
-// Start points of the traversal: the subtrees kFrontierDepth levels below the root (a leaf
-// reached earlier stands for itself), each with its fp32 box grown outward by the
-// threshold (as in TNode), its rightmost leaf and its first leaf position.  A query runs
-// one traversal per start point, so the launch has kFrontier times as many lanes.
+// Start points of the traversal: the subtrees kFrontierDepth levels below the root, each
+// with its fp32 box grown outward by the threshold (as in TNode), its rightmost leaf and
+// its first leaf position.  Slot i follows the path of the bits of i from the root (1 =
+// right); a leaf reached above that depth stands for itself in the slot of its leftmost
+// path, and the other slots below it are empty (first = INT_MAX: no query enters them).
+// A query runs one traversal per start point, so the launch has kFrontier times as many
+// lanes.
 struct Frontier {
-    int count;
+    int count;               // slots in use: kFrontier, or 0 (no traversal)
     int ref[kFrontier];      // internal node index, or ~leaf position
     int rightmost[kFrontier];
-    int first[kFrontier];    // first leaf position (in-order trees; 0 otherwise)
+    int first[kFrontier];    // first leaf position (in-order trees; 0 otherwise); INT_MAX: empty
     float box[kFrontier][6];
 };
 
 template <typename T>
-__global__ void k_frontier(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all, const int *__restrict__ rm_all,
-                           const int *__restrict__ lo_all, int64_t n, double thr, Frontier *__restrict__ fr_all,
-                           const DevCounters *__restrict__ ctr) {
-    const int b = blockIdx.x;
-    if (threadIdx.x != 0) return;
-    Frontier f;
-    f.count = 0;
+__global__ __launch_bounds__(kFrontier) void k_frontier(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
+                                                      const int *__restrict__ rm_all, const int *__restrict__ lo_all, int64_t n,
+                                                      double thr, Frontier *__restrict__ fr_all, const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.x, i = threadIdx.x;     // one thread per slot
+    Frontier &f = fr_all[b];
     if (n < 2 || (ctr[b].flags & DEV_BAD_TREE)) {
-        fr_all[b] = f;
+        if (i == 0) f.count = 0;
         return;
     }
+    if (i == 0) f.count = kFrontier;
     const int64_t rows = 2 * n - 1;
     const T *bb = bb_all + (int64_t)b * rows * 6;
     const int64_t *tree = tree_all + (int64_t)b * rows * 4;
     const int *rm = rm_all + (int64_t)b * (n - 1);
     const int *lo = lo_all + (int64_t)b * (n - 1);
     const bool ordered = !(ctr[b].flags & DEV_UNORDERED);
-    int cur[kFrontier], nc = 1;
-    cur[0] = 0;
+    int ref = 0;            // the root row
+    bool used = true;
     for (int level = 0; level < kFrontierDepth; ++level) {
-        int next[kFrontier], nn = 0;
-        for (int i = 0; i < nc; ++i) {
-            if (cur[i] < 0) { next[nn++] = cur[i]; continue; }
-            const int64_t l = tree[4 * cur[i] + 1], r = tree[4 * cur[i] + 2];
-            next[nn++] = l >= n - 1 ? ~(int)(l - (n - 1)) : (int)l;
-            next[nn++] = r >= n - 1 ? ~(int)(r - (n - 1)) : (int)r;
+        if (ref < 0) {      // a leaf above the frontier depth: only its leftmost slot keeps it
+            used = (i & ((1 << (kFrontierDepth - level)) - 1)) == 0;
+            break;
         }
-        for (int i = 0; i < nn; ++i) cur[i] = next[i];
-        nc = nn;
+        const int64_t c = tree[4 * ref + 1 + ((i >> (kFrontierDepth - 1 - level)) & 1)];
+        ref = c >= n - 1 ? ~(int)(c - (n - 1)) : (int)c;
     }
-    f.count = nc;
-    for (int i = 0; i < nc; ++i) {
-        const int ref = cur[i];
-        const int64_t row = ref >= 0 ? ref : n - 1 + (int64_t)(~ref);
-        f.ref[i] = ref;
-        f.rightmost[i] = ref >= 0 ? rm[ref] : ~ref;
-        f.first[i] = ref >= 0 ? (ordered ? lo[ref] : 0) : ~ref;
-        for (int k = 0; k < 3; ++k) {
-            f.box[i][k] = f32_down((double)bb[6 * row + k] - thr);
-            f.box[i][k + 3] = f32_up((double)bb[6 * row + k + 3] + thr);
-        }
+    f.ref[i] = ref;
+    if (!used) {
+        f.rightmost[i] = 0;
+        f.first[i] = INT_MAX;
+        for (int k = 0; k < 3; ++k) { f.box[i][k] = INFINITY; f.box[i][k + 3] = -INFINITY; }
+        return;
     }
-    fr_all[b] = f;
+    const int64_t row = ref >= 0 ? ref : n - 1 + (int64_t)(~ref);
+    f.rightmost[i] = ref >= 0 ? rm[ref] : ~ref;
+    f.first[i] = ref >= 0 ? (ordered ? lo[ref] : 0) : ~ref;
+    for (int k = 0; k < 3; ++k) {
+        f.box[i][k] = f32_down((double)bb[6 * row + k] - thr);
+        f.box[i][k + 3] = f32_up((double)bb[6 * row + k + 3] + thr);
+    }
 }
 
 // ---- traversal ----------------------------------------------------------------------------
@@ -1254,7 +1256,7 @@ static ccc_status run_query(const Query &q) {
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
     hipLaunchKernelGGL((k_leaf_recs<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, bb, q.tree, n, recs, ctr);
-    hipLaunchKernelGGL((k_frontier<T>), dim3(B), dim3(64), 0, s, bb, q.tree, rm, lo, n, q.thr, frontier, ctr);
+    hipLaunchKernelGGL((k_frontier<T>), dim3(B), dim3(kFrontier), 0, s, bb, q.tree, rm, lo, n, q.thr, frontier, ctr);
     tm.end();
     CCC_LAUNCH_CHECK("tree preparation");
 

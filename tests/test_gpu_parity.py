@@ -542,7 +542,7 @@ def test_deep_tree_reruns_with_the_deep_stack(ccc, oracle):
 
 
 def test_tree_deeper_than_the_deep_stack_raises(ccc, oracle):
-    """Beyond the deep stack (a 40-level spine: 37 pending subtrees below the frontier) the
+    """Beyond the deep stack (a 40-level spine: 33 pending subtrees below the start points) the
     call fails with an error instead of returning a partial pair list."""
     t0, t1 = _cluster(82, seed=22)
     bb, tree = _left_spine_tree(t0, t1, 40)
