@@ -584,7 +584,7 @@ enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
 constexpr int kCcdThreads = 128;
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
 // tiles and a tile adds at most 3 * kCcdThreads.
-constexpr int kFlush = kCcdThreads / 4;
+constexpr int kFlush = kCcdThreads / 2;
 constexpr int kImpStage = kFlush + 3 * kCcdThreads;
 
 template <typename T>
