@@ -209,6 +209,14 @@ template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64
     dv = mk(r.a[3], r.a[4], r.a[5]) - v;
 }
 
+// ld_mtri from vertex records (same values: dv = t1 - t0 in both)
+template <typename T> __device__ __forceinline__ MTri<T> ld_mtri_rec(const T *tv, int64_t i) {
+    MTri<T> m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ld_vrec(tv, 3 * i + k, m.v[k], m.dv[k]);
+    return m;
+}
+
 // gather_mtet from vertex records (same values, so the same tetrahedron bit for bit)
 template <typename T>
 __device__ __forceinline__ MTet<T> gather_mtet_rec(const T *tv, int64_t x, int64_t y, int type) {

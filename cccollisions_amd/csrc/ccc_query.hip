@@ -98,6 +98,37 @@ __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *
     return r;
 }
 
+// block_reserve that also returns the block's range: the thread's slot is base + the
+// returned offset, and the block's slots are [base, base + total).
+template <int NT = kThreads>
+__device__ __forceinline__ unsigned block_reserve_range(unsigned long long *counter, unsigned cnt, unsigned long long &base,
+                                                        unsigned &total) {
+    __shared__ unsigned s_wave[NT / 64 + 1];
+    __shared__ unsigned long long s_base;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
+    unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2) + 8u * lanes_below(b3);
+    unsigned wt = (unsigned)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3));
+    if (lane == 0) s_wave[w] = wt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0;
+        for (int q = 0; q < NT / 64; ++q) {
+            unsigned v = s_wave[q];
+            s_wave[q] = tot;
+            tot += v;
+        }
+        s_wave[NT / 64] = tot;
+        s_base = tot ? atomicAdd(counter, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    base = s_base;
+    total = s_wave[NT / 64];
+    const unsigned r = s_wave[w] + below;
+    __syncthreads();
+    return r;
+}
+
 __device__ __forceinline__ void block_add(unsigned long long *counter, unsigned long long v) {
     __shared__ unsigned long long s_sum;
     if (threadIdx.x == 0) s_sum = 0;
@@ -500,11 +531,12 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 }
 
 // ---- sub-primitive filter -----------------------------------------------------------------
-// MODE 0: moving (collisions_continuous.cu:234-318, :340-396) -> items.
+// MODE 0: moving (collisions_continuous.cu:234-318, :340-396) -> items; t0_all holds the
+//         vertex records [x0 y0 z0 x1 y1 z1] per corner (B x 3n records), t1_all is unused.
 // MODE 1: static with threshold (proximity_static.cu:11-118) -> items.
 // MODE 2: moving, written as (x, y, type) rows (find_minimal_candidates, :321-338).
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
+__global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
                                                       const T *__restrict__ t0_all, const T *__restrict__ t1_all,
                                                       int64_t n, T thr, int64_t *__restrict__ items_all, int64_t capM,
                                                       DevCounters *__restrict__ ctr_all) {
@@ -514,8 +546,8 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
     if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
     const int2 *pairs = pairs_all + (int64_t)b * capK;
-    const T *t0 = t0_all + (int64_t)b * n * 9;
-    const T *t1 = MODE == 1 ? nullptr : t1_all + (int64_t)b * n * 9;
+    const T *t0 = t0_all + (int64_t)b * n * (MODE == 0 ? 18 : 9);
+    const T *t1 = MODE == 2 ? t1_all + (int64_t)b * n * 9 : nullptr;
     int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
     const int64_t ntiles = K > c0 ? (K - c0 + kThreads - 1) / kThreads : 0;
     CCC_FOR_TILES(tile, ntiles) {
@@ -532,23 +564,39 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
                 for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
                 if (!share_vertex(a, c)) mask = static_mask(a, c, thr);
             } else {
-                const MTri<T> a = ld_mtri(t0, t1, x), c = ld_mtri(t0, t1, y);
+                const MTri<T> a = MODE == 0 ? ld_mtri_rec(t0, x) : ld_mtri(t0, t1, x);
+                const MTri<T> c = MODE == 0 ? ld_mtri_rec(t0, y) : ld_mtri(t0, t1, y);
                 if (!share_vertex(a.v, c.v)) mask = moving_mask(a, c);
             }
         }
         const unsigned cnt = (unsigned)__popc(mask);
-        unsigned long long slot = block_reserve(&ctr->items, cnt);
-        while (mask) {
-            const int type = __ffs(mask) - 1;
-            mask &= mask - 1;
-            if (slot < (unsigned long long)capM) {
-                if (MODE == 2) {
-                    items[3 * slot] = x; items[3 * slot + 1] = y; items[3 * slot + 2] = type;
-                } else {
-                    items[slot] = pack_item(x, y, type);
-                }
+        if constexpr (MODE != 2) {
+            // The block's items go through LDS so that the list is written with whole,
+            // consecutive 8-byte stores (one thread's items are consecutive slots).
+            __shared__ int64_t s_items[15 * kThreads];
+            unsigned long long base;
+            unsigned total;
+            unsigned o = block_reserve_range(&ctr->items, cnt, base, total);
+            while (mask) {
+                const int type = __ffs(mask) - 1;
+                mask &= mask - 1;
+                s_items[o++] = pack_item(x, y, type);
             }
-            ++slot;
+            __syncthreads();
+            const unsigned long long room = base < (unsigned long long)capM ? (unsigned long long)capM - base : 0ull;
+            const unsigned lim = room < total ? (unsigned)room : total;
+            for (unsigned k = threadIdx.x; k < lim; k += kThreads) items[base + k] = s_items[k];
+            __syncthreads();
+        } else {
+            unsigned long long slot = block_reserve(&ctr->items, cnt);
+            while (mask) {
+                const int type = __ffs(mask) - 1;
+                mask &= mask - 1;
+                if (slot < (unsigned long long)capM) {
+                    items[3 * slot] = x; items[3 * slot + 1] = y; items[3 * slot + 2] = type;
+                }
+                ++slot;
+            }
         }
     }
 }
@@ -713,9 +761,11 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //      per-workgroup collision lists it took 14 ms per pass on the 130k cloth against ~6
 //      here, where other workgroups' fp64 work hides its atomic latency.
 // Each thread's item of the next tile is loaded while this tile is solved.  Occupancy: the
-// collect mode fits five blocks per CU (<= 96 VGPRs, 18 KB of LDS), the impulse mode four
-// (128 VGPRs, 31 KB).  At five blocks the impulse mode spills about a hundred registers and
-// ran 40% slower; at three (45 KB of LDS, the scatter inlined) 13% slower.
+// collect mode runs five waves per SIMD (<= 96 VGPRs, 10 KB of LDS per block), the impulse
+// mode four (128 VGPRs, 17 KB: eight blocks per CU).  At five waves the impulse mode spills
+// about a hundred registers and ran 40% slower; at three 13% slower.  Staging each wave's
+// query triangle in LDS (so that half the corner reads leave the vector memory path) made
+// the impulse mode spill and took 196-209 ms per step instead of 177.
 struct SolveRec {
     int64_t item;
     double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
@@ -1364,8 +1414,8 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
                                        (const T *)nullptr, n, thr, items, capM, ctr);
                 else
-                    hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
-                                       (const T *)q.t1, n, thr, items, capM, ctr);
+                    hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)tv,
+                                       (const T *)nullptr, n, thr, items, capM, ctr);
                 tm.end();
                 CCC_LAUNCH_CHECK("k_subprim");
                 tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
