@@ -380,8 +380,13 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
                                                            DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all) {
-    __shared__ int stack[DEPTH][kTravThreads];
-    __shared__ int2 stage[kTravThreads / 64][kStage + 128];
+    // Pending right children: with the short stack as 16-bit offsets from the start point's
+    // node (a subtree's nodes lie close together in the trees built here; a farther node
+    // ends the traversal like an overflow, and the call runs again with the deep stack).
+    using StackT = typename std::conditional<(DEPTH <= kStackShort), short, int>::type;
+    __shared__ StackT stack[DEPTH][kTravThreads];
+    __shared__ int stage[kTravThreads / 64][kStage + 128];                // the other triangle of each pair
+    __shared__ int lqid[kTravThreads / 64][64];                         // each lane's query triangle
     __shared__ unsigned short stag[kTravThreads / 64][kStage + 128];   // lane << 10 | rank within the lane
     __shared__ int loff[kTravThreads / 64][64];                         // a flush's offset of each lane
     __shared__ int fill[kTravThreads / 64];
@@ -427,7 +432,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
         alive = alive && fr.first[s] < lim;
     }
     const int lane = tid & 63;
-    int sp = 0, node = fr.ref[s];
+    const int sbase = fr.ref[s];
+    int sp = 0, node = sbase;
     int mine = 0;   // pairs this lane has staged since the last flush
     // Writes the wave's nf staged pairs grouped by lane (lane order, each lane's pairs in the
     // order found), so that the pair list keeps one query's pairs together.  Full wave.
@@ -439,13 +445,17 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             if (lane >= d) incl += v;
         }
         loff[w][lane] = incl - mine;
+        lqid[w][lane] = qid;
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(&ctr->cand, (unsigned long long)nf);
         base = readfirstlane_u64(base);
         for (int j = lane; j < nf; j += 64) {
             const unsigned t = stag[w][j];
             const unsigned long long d = base + (unsigned long long)(loff[w][t >> 10] + (int)(t & 1023u));
-            if (d < (unsigned long long)cap) pairs[d] = stage[w][j];
+            if (d < (unsigned long long)cap) {
+                const int qq = lqid[w][t >> 10], oo = stage[w][j];
+                pairs[d] = make_int2(min(qq, oo), max(qq, oo));
+            }
         }
         mine = 0;
     };
@@ -498,8 +508,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                 const int f = fill[w];
                 int pos = f + (int)below;
                 const unsigned lane_tag = (unsigned)lane << 10;
-                if (oid0 >= 0) { stage[w][pos] = make_int2(min(qid, oid0), max(qid, oid0)); stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
-                if (oid1 >= 0) { stage[w][pos] = make_int2(min(qid, oid1), max(qid, oid1)); stag[w][pos] = (unsigned short)(lane_tag | mine++); }
+                if (oid0 >= 0) { stage[w][pos] = oid0; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
+                if (oid1 >= 0) { stage[w][pos] = oid1; stag[w][pos] = (unsigned short)(lane_tag | mine++); }
                 const int nf = f + (int)tot;
                 if (nf >= kStage) {
                     flush_stage(nf);
@@ -513,12 +523,14 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             const bool tl = ol && nd.d.x >= 0, tr = orr && nd.d.y >= 0;
             if (!tl && !tr) {
                 if (sp == 0) alive = false;
-                else node = stack[--sp][tid];
+                else node = sizeof(StackT) < sizeof(int) ? sbase + (int)stack[--sp][tid] : (int)stack[--sp][tid];
             } else {
                 node = tl ? nd.d.x : nd.d.y;
                 if (tl && tr) {
-                    if (sp >= DEPTH) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
-                    else stack[sp++][tid] = nd.d.y;
+                    const int push = sizeof(StackT) < sizeof(int) ? nd.d.y - sbase : nd.d.y;
+                    const bool fits = sizeof(StackT) == sizeof(int) || (push >= -32768 && push <= 32767);
+                    if (sp >= DEPTH || !fits) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
+                    else stack[sp++][tid] = (StackT)push;
                 }
             }
         }

@@ -551,6 +551,61 @@ def test_tree_deeper_than_the_deep_stack_raises(ccc, oracle):
         ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
 
 
+def _renumbered_internal_tree(rb, rt, seed):
+    """The same tree with its internal rows other than the root (row 0) renumbered at random:
+    the nodes of a subtree no longer lie close together in the row order."""
+    n = (rt.shape[0] + 1) // 2
+    rng = np.random.default_rng(seed)
+    remap = np.arange(2 * n - 1)                     # old row -> new row
+    remap[1:n - 1] = 1 + rng.permutation(n - 2)
+    bb, tree = np.empty_like(rb), np.empty_like(rt)
+    bb[remap], tree[remap] = rb, rt
+    tree[:, 0] = np.where(tree[:, 0] >= 0, remap[np.maximum(tree[:, 0], 0)], tree[:, 0])
+    tree[:n - 1, 1:3] = remap[tree[:n - 1, 1:3]]
+    return bb, tree
+
+
+def test_far_apart_subtree_nodes_take_the_deep_stack(ccc, oracle):
+    """The short traversal stack holds 16-bit offsets from the start point's node; in a tree
+    of 40,000 leaves with internal rows numbered at random those offsets do not fit, the
+    traversal runs again with the deep stack, and the pairs still equal the oracle's."""
+    n = 40000
+    rng = np.random.default_rng(31)
+    c = rng.random((n, 3))
+    # triangles numbered in Z order of their centres: the reference's tree over triangle
+    # indices then has tight boxes
+    cell = np.minimum((c * 1024).astype(np.int64), 1023)
+    key = np.zeros(n, np.int64)
+    for bit in range(10):
+        for a in range(3):
+            key |= ((cell[:, a] >> bit) & 1) << (3 * bit + a)
+    t0 = c[np.argsort(key, kind="stable")][:, None, :] + 0.01 * rng.standard_normal((n, 3, 3))
+    t1 = t0 + 0.005 * rng.standard_normal((n, 3, 3))
+    rb, rt, _, _ = oracle.bvh(t0, t1)
+    bb, tree = _renumbered_internal_tree(rb, rt, 4)
+    # premise: below the start points (depth 7), internal rows lie more than 32767 rows
+    # away from their start point's row
+    level, far = [0], 0
+    for _ in range(7):
+        level = [c for r in level for c in (tree[r, 1], tree[r, 2]) if r < n - 1] + [r for r in level if r >= n - 1]
+    for r0 in level:
+        todo = [r0]
+        while todo:
+            r = todo.pop()
+            if r < n - 1:
+                far = max(far, abs(int(r) - int(r0)))
+                todo += [tree[r, 1], tree[r, 2]]
+    assert far > 32767
+    pairs, _ = oracle.traverse(bb, tree, 0.0)
+    np.testing.assert_array_equal(sorted_rows(pairs), sorted_rows(oracle.traverse(rb, rt, 0.0)[0]))
+    coll, roots, counts = oracle.ccd(t0, t1, pairs)
+    assert counts["candidates"] > n and counts["collisions"] > 0
+    mc = int(np.ceil(counts["candidates"] / n)) + 8
+    gc, gr = ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), mc, 4096)
+    assert ccc.last_counts()[0][0] == counts["candidates"]
+    _compare_collisions(_np(gc)[0], _np(gr)[0], coll, roots)
+
+
 def test_single_triangle_queries(ccc, oracle):
     """N = 1 (the reference cannot build it): a one-leaf tree, no pairs, padded outputs and
     zero impulses from every query."""
