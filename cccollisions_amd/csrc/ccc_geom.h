@@ -209,12 +209,54 @@ template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64
     dv = mk(r.a[3], r.a[4], r.a[5]) - v;
 }
 
-// ld_mtri from vertex records (same values: dv = t1 - t0 in both)
-template <typename T> __device__ __forceinline__ MTri<T> ld_mtri_rec(const T *tv, int64_t i) {
-    MTri<T> m;
+// share_vertex + moving_mask of triangles x and y, one coordinate axis at a time from the
+// vertex records (so that only one axis of the twelve corner positions is live at once).
+// Every step is exact -- the swept end t0 + (t1 - t0), fmin / fmax, the comparisons -- so
+// per-axis results combine to the same answer: a shared vertex is equal on all three axes,
+// and boxes overlap when their intervals overlap on all three.
+template <typename T> __device__ __forceinline__ uint32_t moving_mask_axes(const T *tv, int64_t x, int64_t y) {
+    uint32_t eq = 0x1ffu, m = 0x7fffu;    // bit 3i + j: corner i of x equals corner j of y
+    const T *rx = tv + 18 * x, *ry = tv + 18 * y;
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        T lo[6], hi[6], v0[6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) ld_vrec(tv, 3 * i + k, m.v[k], m.dv[k]);
-    return m;
+        for (int k = 0; k < 6; ++k) {
+            const T *r = (k < 3 ? rx : ry) + 6 * (k % 3);
+            const T v = r[d], w = v + (r[3 + d] - v);
+            v0[k] = v;
+            lo[k] = tmin(v, w);
+            hi[k] = tmax(v, w);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (!(v0[i] == v0[3 + j])) eq &= ~(1u << (3 * i + j));
+        // edge intervals E = {(0,1),(0,2),(1,2)} and the face interval of each triangle
+        T elo[6], ehi[6];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int a = 3 * s + edge_first(e), c = 3 * s + edge_second(e);
+                elo[3 * s + e] = tmin(lo[a], lo[c]);
+                ehi[3 * s + e] = tmax(hi[a], hi[c]);
+            }
+        const T flo0 = tmin(tmin(lo[0], lo[1]), lo[2]), fhi0 = tmax(tmax(hi[0], hi[1]), hi[2]);
+        const T flo1 = tmin(tmin(lo[3], lo[4]), lo[5]), fhi1 = tmax(tmax(hi[3], hi[4]), hi[5]);
+        auto ov = [](T l1, T h1, T l2, T h2) { return (l1 <= (h2 + (T)0)) && (h1 >= (l2 - (T)0)); };
+        uint32_t ma = 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) ma |= ov(elo[t / 3], ehi[t / 3], elo[3 + t % 3], ehi[3 + t % 3]) ? (1u << t) : 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ma |= ov(flo0, fhi0, lo[3 + k], hi[3 + k]) ? (1u << (9 + k)) : 0u;
+            ma |= ov(lo[k], hi[k], flo1, fhi1) ? (1u << (12 + k)) : 0u;
+        }
+        m &= ma;
+    }
+    return eq ? 0u : m;
 }
 
 // gather_mtet from vertex records (same values, so the same tetrahedron bit for bit)

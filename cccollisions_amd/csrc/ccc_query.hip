@@ -543,10 +543,9 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 }
 
 // ---- sub-primitive filter -----------------------------------------------------------------
-// MODE 0: moving (collisions_continuous.cu:234-318, :340-396) -> items; t0_all holds the
-//         vertex records [x0 y0 z0 x1 y1 z1] per corner (B x 3n records), t1_all is unused.
 // MODE 1: static with threshold (proximity_static.cu:11-118) -> items.
 // MODE 2: moving, written as (x, y, type) rows (find_minimal_candidates, :321-338).
+// (Moving pairs -> items, collisions_continuous.cu:234-318, :340-396: k_subprim_moving.)
 template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
                                                       const T *__restrict__ t0_all, const T *__restrict__ t1_all,
@@ -558,7 +557,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict_
     int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
     if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
     const int2 *pairs = pairs_all + (int64_t)b * capK;
-    const T *t0 = t0_all + (int64_t)b * n * (MODE == 0 ? 18 : 9);
+    static_assert(MODE == 1 || MODE == 2, "moving pairs -> items: k_subprim_moving");
+    const T *t0 = t0_all + (int64_t)b * n * 9;
     const T *t1 = MODE == 2 ? t1_all + (int64_t)b * n * 9 : nullptr;
     int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
     const int64_t ntiles = K > c0 ? (K - c0 + kThreads - 1) / kThreads : 0;
@@ -576,13 +576,12 @@ __global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict_
                 for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
                 if (!share_vertex(a, c)) mask = static_mask(a, c, thr);
             } else {
-                const MTri<T> a = MODE == 0 ? ld_mtri_rec(t0, x) : ld_mtri(t0, t1, x);
-                const MTri<T> c = MODE == 0 ? ld_mtri_rec(t0, y) : ld_mtri(t0, t1, y);
+                const MTri<T> a = ld_mtri(t0, t1, x), c = ld_mtri(t0, t1, y);
                 if (!share_vertex(a.v, c.v)) mask = moving_mask(a, c);
             }
         }
         const unsigned cnt = (unsigned)__popc(mask);
-        if constexpr (MODE != 2) {
+        if constexpr (MODE == 1) {
             // The block's items go through LDS so that the list is written with whole,
             // consecutive 8-byte stores (one thread's items are consecutive slots).
             __shared__ int64_t s_items[15 * kThreads];
@@ -610,6 +609,91 @@ __global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict_
                 ++slot;
             }
         }
+    }
+}
+
+// Exclusive block-wide scan of cnt (< 16) over the NT threads in thread order, without a
+// reservation: the thread's offset; the block's sum in `total`.  Every thread calls it.
+template <int NT = kThreads>
+__device__ __forceinline__ unsigned block_scan_excl(unsigned cnt, unsigned &total) {
+    __shared__ unsigned s_part[NT / 64 + 1];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
+    unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2) + 8u * lanes_below(b3);
+    if (lane == 0) s_part[w] = (unsigned)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3));
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) {
+        const unsigned v = s_part[q];
+        if (q < w) before += v;
+        tot += v;
+    }
+    __syncthreads();
+    total = tot;
+    return before + below;
+}
+
+// MODE 0 of k_subprim with PP pairs per thread and one reservation per PP * kThreads pairs
+// (the single device-wide item counter is the contended resource: every reservation is a
+// returning atomic on one address).  Items stay in pair order: pair u * kThreads + t of the
+// tile comes after pairs u' * kThreads + t' < u * kThreads + t.  The sub-primitive mask is
+// computed one axis at a time from the vertex records (moving_mask_axes), which keeps the
+// kernel's registers low enough for PP pairs per thread.
+template <typename T, int PP>
+__global__ __launch_bounds__(kThreads) void k_subprim_moving(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0,
+                                                             int64_t c1, const T *__restrict__ tv_all, int64_t n,
+                                                             int64_t *__restrict__ items_all, int64_t capM,
+                                                             DevCounters *__restrict__ ctr_all) {
+    static_assert(PP >= 1 && PP * kThreads <= 4096, "pair index and type fit 16 bits");
+    __shared__ unsigned short s_ent[15 * PP * kThreads];    // (pair in tile) << 4 | type
+    __shared__ int2 s_pair[PP * kThreads];
+    __shared__ unsigned long long s_base;
+    const int b = blockIdx.y, tid = threadIdx.x;
+    DevCounters *ctr = ctr_all + b;
+    const unsigned long long kc = ctr->cand;
+    int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
+    if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
+    const int2 *pairs = pairs_all + (int64_t)b * capK;
+    const T *tv = tv_all + (int64_t)b * n * 18;
+    int64_t *items = items_all + (int64_t)b * capM;
+    constexpr int kTile = PP * kThreads;
+    const int64_t ntiles = K > c0 ? (K - c0 + kTile - 1) / kTile : 0;
+    CCC_FOR_TILES(tile, ntiles) {
+        unsigned tot = 0;
+#pragma unroll
+        for (int u = 0; u < PP; ++u) {
+            const int64_t i = c0 + tile * kTile + u * kThreads + tid;
+            uint32_t mask = 0;
+            int x = 0, y = 0;
+            if (i < K) {
+                const int2 pr = pairs[i];
+                x = pr.x;
+                y = pr.y;
+                mask = moving_mask_axes(tv, x, y);
+            }
+            s_pair[u * kThreads + tid] = make_int2(x, y);
+            unsigned t;
+            const unsigned o = tot + block_scan_excl((unsigned)__popc(mask), t);
+            tot += t;
+            unsigned k = o;
+            while (mask) {
+                const int type = __ffs(mask) - 1;
+                mask &= mask - 1;
+                s_ent[k++] = (unsigned short)(((u * kThreads + tid) << 4) | type);
+            }
+        }
+        if (tid == 0) s_base = tot ? atomicAdd(&ctr->items, (unsigned long long)tot) : 0ull;
+        __syncthreads();
+        const unsigned long long base = s_base;
+        const unsigned long long room = base < (unsigned long long)capM ? (unsigned long long)capM - base : 0ull;
+        const unsigned lim = room < tot ? (unsigned)room : tot;
+        for (unsigned k = tid; k < lim; k += kThreads) {
+            const unsigned e = s_ent[k];
+            const int2 pr = s_pair[e >> 4];
+            items[base + k] = pack_item(pr.x, pr.y, (int)(e & 15u));
+        }
+        __syncthreads();
     }
 }
 
@@ -1244,6 +1328,7 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 // Pairs per narrow-phase chunk: the item list holds at most 15 items per pair of a chunk,
 // 4 GiB per batch element at this size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
+constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim_moving
 
 // Grid of the tile-looping kernels: one block per tile up to 128 blocks per CU, each block
 // then walking several tiles.  Measured on the 130k cloth (k_ccd per step): 4 blocks per CU
@@ -1410,6 +1495,7 @@ static ccc_status run_query(const Query &q) {
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         const unsigned gC = persistent_grid(q.ctx, chunk, B);
+        const unsigned gS = persistent_grid(q.ctx, (chunk + kSubprimPairs - 1) / kSubprimPairs, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
         const unsigned gCcd = persistent_grid(q.ctx, capM, B, kCcdThreads);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
@@ -1426,8 +1512,8 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
                                        (const T *)nullptr, n, thr, items, capM, ctr);
                 else
-                    hipLaunchKernelGGL((k_subprim<T, 0>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)tv,
-                                       (const T *)nullptr, n, thr, items, capM, ctr);
+                    hipLaunchKernelGGL((k_subprim_moving<T, kSubprimPairs>), dim3(gS, B), dim3(kThreads), 0, s, pairs, capK, c0, c1,
+                                       (const T *)tv, n, items, capM, ctr);
                 tm.end();
                 CCC_LAUNCH_CHECK("k_subprim");
                 tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
