@@ -867,6 +867,7 @@ struct SolveRec {
     double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
 };
 constexpr int kRootCap = 3 * kCcdThreads;
+constexpr int kCollectFlush = 256;
 
 __device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
     if (code == CUBIC_QUADRATIC) { r.v[0] = st.b; r.v[1] = st.c; r.v[2] = st.d; r.v[3] = 0; }
@@ -899,8 +900,13 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * kCcdThreads + sizeof(T) * kRootCap);
     __shared__ int s_cnt[5];
     __shared__ int s_nroot;
-    __shared__ int64_t s_item[MODE == MODE_IMPULSE ? kImpStage : 1];
-    __shared__ T s_root[MODE == MODE_IMPULSE ? kImpStage : 1];
+    // accepted collisions wait here: impulse mode until kFlush of them are scattered,
+    // collect mode until kCollectFlush of them are written as output rows with one
+    // reservation (the row counter is one contended device address)
+    constexpr int kStageCap = MODE == MODE_IMPULSE ? kImpStage : kCollectFlush + 3 * kCcdThreads;
+    __shared__ int64_t s_item[kStageCap];
+    __shared__ T s_root[kStageCap];
+    __shared__ unsigned long long s_obase;
     __shared__ int s_fill;
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
@@ -970,6 +976,26 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
             tab.drain(tid, dv, dx, cnt);
             __syncthreads();
         }
+    };
+
+    // Collect mode: the staged collisions [0, f) to output rows (rows past out_cap are
+    // counted, not written).  Every thread calls it.
+    auto write_rows = [&](int f) {
+        if (tid == 0) s_obase = f ? atomicAdd(&ctr->coll, (unsigned long long)f) : 0ull;
+        __syncthreads();
+        const unsigned long long ob = s_obase;
+        int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
+        for (int j = tid; j < f; j += kCcdThreads) {
+            const unsigned long long slot = ob + (unsigned long long)j;
+            if (slot < (unsigned long long)out_cap) {
+                int64_t x, y;
+                int type;
+                unpack_item(s_item[j], x, y, type);
+                coll[3 * slot] = x; coll[3 * slot + 1] = y; coll[3 * slot + 2] = type;
+                roots_all[(int64_t)b * out_cap + slot] = s_root[j];
+            }
+        }
+        __syncthreads();
     };
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
@@ -1048,17 +1074,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
                 const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
                 ok = filter_collision(m, type, root);
             }
-            if (MODE == MODE_COLLECT) {
-                const unsigned long long slot = block_reserve<kCcdThreads>(&ctr->coll, ok ? 1u : 0u);
-                if (ok && slot < (unsigned long long)out_cap) {
-                    int64_t x, y;
-                    int type;
-                    unpack_item(rit, x, y, type);
-                    int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
-                    coll[3 * slot] = x; coll[3 * slot + 1] = y; coll[3 * slot + 2] = type;
-                    roots_all[(int64_t)b * out_cap + slot] = root;
-                }
-            } else if (ok) {
+            if (ok) {
                 const int p = atomicAdd(&s_fill, 1);
                 s_item[p] = rit;
                 s_root[p] = root;
@@ -1080,6 +1096,14 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
                 __syncthreads();
                 if (tid == 0) s_fill = f - done;
             }
+        } else {
+            // ---- D (collect): write the staged rows once kCollectFlush have gathered
+            __syncthreads();
+            const int f = s_fill;
+            if (f >= kCollectFlush) {
+                write_rows(f);
+                if (tid == 0) s_fill = 0;
+            }
         }
         __syncthreads();
     }
@@ -1087,6 +1111,8 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
         const int f = s_fill;
         flush(f);
         if (tid == 0) ncoll += (unsigned long long)f;
+    } else {
+        write_rows(s_fill);
     }
     block_add(&ctr->roots, nroots);
     if (MODE == MODE_IMPULSE) block_add(&ctr->coll, ncoll);
