@@ -230,8 +230,11 @@ def serving_section(ccc, torch, scenes, args):
         t0 = time.perf_counter()
         reqs = []
         for a, s in zip(arrivals, sim_ids):
-            while time.perf_counter() - t0 < a:
-                time.sleep(min(0.002, a - (time.perf_counter() - t0)))
+            while True:
+                left = a - (time.perf_counter() - t0)
+                if left <= 0:
+                    break
+                time.sleep(min(0.002, left))
             reqs.append(server.submit(int(s), arrival=server.now()))
         server.close()
         span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
