@@ -71,25 +71,6 @@ template <typename T> __device__ __forceinline__ Box<T> tri_box(V3<T> a, V3<T> b
     return Box<T>{mk(tmin(a.x, tmin(b.x, c.x)), tmin(a.y, tmin(b.y, c.y)), tmin(a.z, tmin(b.z, c.z))),
                   mk(tmax(a.x, tmax(b.x, c.x)), tmax(a.y, tmax(b.y, c.y)), tmax(a.z, tmax(b.z, c.z)))};
 }
-// figures.cu:346-351 swept box of one moving vertex (v, v + dv)
-template <typename T> __device__ __forceinline__ Box<T> swept_vbox(V3<T> v, V3<T> dv) {
-    V3<T> w = v + dv;
-    return Box<T>{mk(tmin(v.x, w.x), tmin(v.y, w.y), tmin(v.z, w.z)), mk(tmax(v.x, w.x), tmax(v.y, w.y), tmax(v.z, w.z))};
-}
-
-// A triangle and its motion over the step: figures.cu:295-314 (dv = v(t1) - v(t0)).
-template <typename T> struct MTri { V3<T> v[3]; V3<T> dv[3]; };
-
-template <typename T> __device__ __forceinline__ MTri<T> ld_mtri(const T *t0, const T *t1, int64_t i) {
-    MTri<T> m;
-    const T *a = t0 + 9 * i, *b = t1 + 9 * i;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        m.v[k] = ld3(a + 3 * k);
-        m.dv[k] = ld3(b + 3 * k) - m.v[k];
-    }
-    return m;
-}
 
 // figures.cu:527-544 shareVertex: exact coincidence of any t0 vertices.
 template <typename T> __device__ __forceinline__ bool share_vertex(const V3<T> *a, const V3<T> *b) {
@@ -128,41 +109,11 @@ __device__ __forceinline__ void type_corner(int type, int k, int &side, int &idx
     }
 }
 
-// Swept box of a sub-primitive of a moving triangle (figures.cu:346-364):
-// kind 0: vertex i; kind 1: edge e; kind 2: whole triangle.
-template <typename T> __device__ __forceinline__ Box<T> mtri_edge_box(const MTri<T> &t, int e) {
-    return unite(swept_vbox(t.v[edge_first(e)], t.dv[edge_first(e)]), swept_vbox(t.v[edge_second(e)], t.dv[edge_second(e)]));
-}
-template <typename T> __device__ __forceinline__ Box<T> mtri_box(const MTri<T> &t) {
-    return unite(unite(swept_vbox(t.v[0], t.dv[0]), swept_vbox(t.v[1], t.dv[1])), swept_vbox(t.v[2], t.dv[2]));
-}
 // Static sub-primitive boxes (figures.cu:58-67; proximity_static.cu:11-98).
 template <typename T> __device__ __forceinline__ Box<T> tri_edge_box(const V3<T> *v, int e) {
     return unite(point_box(v[edge_first(e)]), point_box(v[edge_second(e)]));
 }
 
-// 15-bit mask of sub-primitive pairs whose swept boxes overlap
-// (collisions_continuous.cu:234-318, threshold 0).
-template <typename T> __device__ __forceinline__ uint32_t moving_mask(const MTri<T> &a, const MTri<T> &b) {
-    Box<T> ea[3], eb[3], va[3], vb[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        ea[i] = mtri_edge_box(a, i);
-        eb[i] = mtri_edge_box(b, i);
-        va[i] = swept_vbox(a.v[i], a.dv[i]);
-        vb[i] = swept_vbox(b.v[i], b.dv[i]);
-    }
-    Box<T> fa = mtri_box(a), fb = mtri_box(b);
-    uint32_t m = 0;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) m |= overlap(ea[t / 3], eb[t % 3], (T)0) ? (1u << t) : 0u;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        m |= overlap(fa, vb[k], (T)0) ? (1u << (9 + k)) : 0u;
-        m |= overlap(va[k], fb, (T)0) ? (1u << (12 + k)) : 0u;
-    }
-    return m;
-}
 // Same filter for static triangles with a threshold (proximity_static.cu:11-98).
 template <typename T> __device__ __forceinline__ uint32_t static_mask(const V3<T> *a, const V3<T> *b, T thr) {
     Box<T> ea[3], eb[3];
@@ -209,11 +160,15 @@ template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64
     dv = mk(r.a[3], r.a[4], r.a[5]) - v;
 }
 
-// share_vertex + moving_mask of triangles x and y, one coordinate axis at a time from the
-// vertex records (so that only one axis of the twelve corner positions is live at once).
-// Every step is exact -- the swept end t0 + (t1 - t0), fmin / fmax, the comparisons -- so
-// per-axis results combine to the same answer: a shared vertex is equal on all three axes,
-// and boxes overlap when their intervals overlap on all three.
+// Sub-primitive filter of a moving pair (collisions_continuous.cu:234-318, threshold 0): the
+// 15-bit mask of sub-primitive pairs whose swept boxes overlap (the swept box of a vertex is
+// the box of v and v + dv, of an edge or face the union of its vertices' boxes,
+// figures.cu:295-314 / :346-364), or 0 when the triangles share a t0 vertex exactly
+// (shareVertex, figures.cu:527-544).  Triangles x and y come from the vertex records and are
+// processed one coordinate axis at a time, so that only one axis of the twelve corner
+// positions is live at once.  Every step is exact -- the swept end t0 + (t1 - t0), fmin /
+// fmax, the comparisons -- so per-axis results combine to the same answer: a shared vertex is
+// equal on all three axes, and boxes overlap when their intervals overlap on all three.
 template <typename T> __device__ __forceinline__ uint32_t moving_mask_axes(const T *tv, int64_t x, int64_t y) {
     uint32_t eq = 0x1ffu, m = 0x7fffu;    // bit 3i + j: corner i of x equals corner j of y
     const T *rx = tv + 18 * x, *ry = tv + 18 * y;

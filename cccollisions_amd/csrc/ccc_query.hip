@@ -98,37 +98,6 @@ __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *
     return r;
 }
 
-// block_reserve that also returns the block's range: the thread's slot is base + the
-// returned offset, and the block's slots are [base, base + total).
-template <int NT = kThreads>
-__device__ __forceinline__ unsigned block_reserve_range(unsigned long long *counter, unsigned cnt, unsigned long long &base,
-                                                        unsigned &total) {
-    __shared__ unsigned s_wave[NT / 64 + 1];
-    __shared__ unsigned long long s_base;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u), b3 = __ballot(cnt & 8u);
-    unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2) + 8u * lanes_below(b3);
-    unsigned wt = (unsigned)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3));
-    if (lane == 0) s_wave[w] = wt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned tot = 0;
-        for (int q = 0; q < NT / 64; ++q) {
-            unsigned v = s_wave[q];
-            s_wave[q] = tot;
-            tot += v;
-        }
-        s_wave[NT / 64] = tot;
-        s_base = tot ? atomicAdd(counter, (unsigned long long)tot) : 0ull;
-    }
-    __syncthreads();
-    base = s_base;
-    total = s_wave[NT / 64];
-    const unsigned r = s_wave[w] + below;
-    __syncthreads();
-    return r;
-}
-
 __device__ __forceinline__ void block_add(unsigned long long *counter, unsigned long long v) {
     __shared__ unsigned long long s_sum;
     if (threadIdx.x == 0) s_sum = 0;
@@ -543,75 +512,18 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 }
 
 // ---- sub-primitive filter -----------------------------------------------------------------
-// MODE 1: static with threshold (proximity_static.cu:11-118) -> items.
-// MODE 2: moving, written as (x, y, type) rows (find_minimal_candidates, :321-338).
-// (Moving pairs -> items, collisions_continuous.cu:234-318, :340-396: k_subprim_moving.)
-template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads, 4) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
-                                                      const T *__restrict__ t0_all, const T *__restrict__ t1_all,
-                                                      int64_t n, T thr, int64_t *__restrict__ items_all, int64_t capM,
-                                                      DevCounters *__restrict__ ctr_all) {
-    const int b = blockIdx.y;
-    DevCounters *ctr = ctr_all + b;
-    const unsigned long long kc = ctr->cand;
-    int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
-    if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
-    const int2 *pairs = pairs_all + (int64_t)b * capK;
-    static_assert(MODE == 1 || MODE == 2, "moving pairs -> items: k_subprim_moving");
-    const T *t0 = t0_all + (int64_t)b * n * 9;
-    const T *t1 = MODE == 2 ? t1_all + (int64_t)b * n * 9 : nullptr;
-    int64_t *items = items_all + (int64_t)b * capM * (MODE == 2 ? 3 : 1);
-    const int64_t ntiles = K > c0 ? (K - c0 + kThreads - 1) / kThreads : 0;
-    CCC_FOR_TILES(tile, ntiles) {
-        const int64_t i = c0 + tile * kThreads + threadIdx.x;
-        uint32_t mask = 0;
-        int x = 0, y = 0;
-        if (i < K) {
-            const int2 pr = pairs[i];
-            x = pr.x;
-            y = pr.y;
-            if (MODE == 1) {
-                V3<T> a[3], c[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) { a[k] = ld3(t0 + 9 * (int64_t)x + 3 * k); c[k] = ld3(t0 + 9 * (int64_t)y + 3 * k); }
-                if (!share_vertex(a, c)) mask = static_mask(a, c, thr);
-            } else {
-                const MTri<T> a = ld_mtri(t0, t1, x), c = ld_mtri(t0, t1, y);
-                if (!share_vertex(a.v, c.v)) mask = moving_mask(a, c);
-            }
-        }
-        const unsigned cnt = (unsigned)__popc(mask);
-        if constexpr (MODE == 1) {
-            // The block's items go through LDS so that the list is written with whole,
-            // consecutive 8-byte stores (one thread's items are consecutive slots).
-            __shared__ int64_t s_items[15 * kThreads];
-            unsigned long long base;
-            unsigned total;
-            unsigned o = block_reserve_range(&ctr->items, cnt, base, total);
-            while (mask) {
-                const int type = __ffs(mask) - 1;
-                mask &= mask - 1;
-                s_items[o++] = pack_item(x, y, type);
-            }
-            __syncthreads();
-            const unsigned long long room = base < (unsigned long long)capM ? (unsigned long long)capM - base : 0ull;
-            const unsigned lim = room < total ? (unsigned)room : total;
-            for (unsigned k = threadIdx.x; k < lim; k += kThreads) items[base + k] = s_items[k];
-            __syncthreads();
-        } else {
-            unsigned long long slot = block_reserve(&ctr->items, cnt);
-            while (mask) {
-                const int type = __ffs(mask) - 1;
-                mask &= mask - 1;
-                if (slot < (unsigned long long)capM) {
-                    items[3 * slot] = x; items[3 * slot + 1] = y; items[3 * slot + 2] = type;
-                }
-                ++slot;
-            }
-        }
-    }
-}
-
+// Candidate pairs -> the pairs of sub-primitives (9 edge-edge, 6 vertex-face) whose boxes
+// overlap, with PP pairs per thread and one reservation per PP * kThreads pairs (the item
+// counter is one contended device address: every reservation is a returning atomic on it).
+//   SUB_MOVING: swept boxes (collisions_continuous.cu:234-318, :340-396) -> packed items;
+//   SUB_STATIC: boxes grown by the threshold (proximity_static.cu:11-118) -> packed items;
+//   SUB_MINCOL: swept boxes, written as (x, y, type) rows (find_minimal_candidates, :321-338).
+// Items stay in pair order: pair u * kThreads + t of a tile comes after every pair
+// u' * kThreads + t' < u * kThreads + t.  The moving masks are computed one axis at a time
+// from the vertex records (moving_mask_axes), which keeps the registers low enough for PP
+// pairs per thread.  A tile's items are assembled in LDS (a 2-byte entry per item: pair in
+// tile and type) and written as one contiguous run.
+enum { SUB_MOVING = 0, SUB_STATIC = 1, SUB_MINCOL = 2 };
 // Exclusive block-wide scan of cnt (< 16) over the NT threads in thread order, without a
 // reservation: the thread's offset; the block's sum in `total`.  Every thread calls it.
 template <int NT = kThreads>
@@ -634,17 +546,11 @@ __device__ __forceinline__ unsigned block_scan_excl(unsigned cnt, unsigned &tota
     return before + below;
 }
 
-// MODE 0 of k_subprim with PP pairs per thread and one reservation per PP * kThreads pairs
-// (the single device-wide item counter is the contended resource: every reservation is a
-// returning atomic on one address).  Items stay in pair order: pair u * kThreads + t of the
-// tile comes after pairs u' * kThreads + t' < u * kThreads + t.  The sub-primitive mask is
-// computed one axis at a time from the vertex records (moving_mask_axes), which keeps the
-// kernel's registers low enough for PP pairs per thread.
-template <typename T, int PP>
-__global__ __launch_bounds__(kThreads) void k_subprim_moving(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0,
-                                                             int64_t c1, const T *__restrict__ tv_all, int64_t n,
-                                                             int64_t *__restrict__ items_all, int64_t capM,
-                                                             DevCounters *__restrict__ ctr_all) {
+template <typename T, int KIND, int PP>
+__global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
+                                                      const T *__restrict__ src_all, int64_t n, T thr,
+                                                      int64_t *__restrict__ out_all, int64_t capM,
+                                                      DevCounters *__restrict__ ctr_all) {
     static_assert(PP >= 1 && PP * kThreads <= 4096, "pair index and type fit 16 bits");
     __shared__ unsigned short s_ent[15 * PP * kThreads];    // (pair in tile) << 4 | type
     __shared__ int2 s_pair[PP * kThreads];
@@ -655,8 +561,9 @@ __global__ __launch_bounds__(kThreads) void k_subprim_moving(const int2 *__restr
     int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
     if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
     const int2 *pairs = pairs_all + (int64_t)b * capK;
-    const T *tv = tv_all + (int64_t)b * n * 18;
-    int64_t *items = items_all + (int64_t)b * capM;
+    // SUB_STATIC: the t0 soup; otherwise the vertex records [x0 y0 z0 x1 y1 z1] per corner
+    const T *src = src_all + (int64_t)b * n * (KIND == SUB_STATIC ? 9 : 18);
+    int64_t *out = out_all + (int64_t)b * capM * (KIND == SUB_MINCOL ? 3 : 1);
     constexpr int kTile = PP * kThreads;
     const int64_t ntiles = K > c0 ? (K - c0 + kTile - 1) / kTile : 0;
     CCC_FOR_TILES(tile, ntiles) {
@@ -670,13 +577,19 @@ __global__ __launch_bounds__(kThreads) void k_subprim_moving(const int2 *__restr
                 const int2 pr = pairs[i];
                 x = pr.x;
                 y = pr.y;
-                mask = moving_mask_axes(tv, x, y);
+                if (KIND == SUB_STATIC) {
+                    V3<T> a[3], c[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) { a[k] = ld3(src + 9 * (int64_t)x + 3 * k); c[k] = ld3(src + 9 * (int64_t)y + 3 * k); }
+                    if (!share_vertex(a, c)) mask = static_mask(a, c, thr);
+                } else {
+                    mask = moving_mask_axes(src, x, y);
+                }
             }
             s_pair[u * kThreads + tid] = make_int2(x, y);
             unsigned t;
-            const unsigned o = tot + block_scan_excl((unsigned)__popc(mask), t);
+            unsigned k = tot + block_scan_excl((unsigned)__popc(mask), t);
             tot += t;
-            unsigned k = o;
             while (mask) {
                 const int type = __ffs(mask) - 1;
                 mask &= mask - 1;
@@ -691,7 +604,12 @@ __global__ __launch_bounds__(kThreads) void k_subprim_moving(const int2 *__restr
         for (unsigned k = tid; k < lim; k += kThreads) {
             const unsigned e = s_ent[k];
             const int2 pr = s_pair[e >> 4];
-            items[base + k] = pack_item(pr.x, pr.y, (int)(e & 15u));
+            const unsigned long long slot = base + k;
+            if (KIND == SUB_MINCOL) {
+                out[3 * slot] = pr.x; out[3 * slot + 1] = pr.y; out[3 * slot + 2] = (int64_t)(e & 15u);
+            } else {
+                out[slot] = pack_item(pr.x, pr.y, (int)(e & 15u));
+            }
         }
         __syncthreads();
     }
@@ -1354,7 +1272,7 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 // Pairs per narrow-phase chunk: the item list holds at most 15 items per pair of a chunk,
 // 4 GiB per batch element at this size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
-constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim_moving
+constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim
 
 // Grid of the tile-looping kernels: one block per tile up to 128 blocks per CU, each block
 // then walking several tiles.  Measured on the 130k cloth (k_ccd per step): 4 blocks per CU
@@ -1510,8 +1428,15 @@ static ccc_status run_query(const Query &q) {
         tm.end();
         CCC_LAUNCH_CHECK("k_tri_edge");
     } else if (q.narrow == Narrow::MINCOL) {
-        CCC_TIMED(tm, "mincol", hipLaunchKernelGGL((k_subprim<T, 2>), dim3(gK, B), dim3(kThreads), 0, s, pairs, capK, (int64_t)0,
-                                                   capK, (const T *)q.t0, (const T *)q.t1, n, thr, q.out_rows, q.out_cap, ctr));
+        T *tv = ws.get_as<T>(B * n * 18);
+        if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        const unsigned gS = persistent_grid(q.ctx, (kmax + kSubprimPairs - 1) / kSubprimPairs, B);
+        tm.begin("mincol");
+        hipLaunchKernelGGL((k_vertex_records<T>), dim3(grid_for(3 * n, 256), B), dim3(256), 0, s, (const T *)q.t0,
+                           (const T *)q.t1, 3 * n, tv);
+        hipLaunchKernelGGL((k_subprim<T, SUB_MINCOL, kSubprimPairs>), dim3(gS, B), dim3(kThreads), 0, s, pairs, capK,
+                           (int64_t)0, capK, (const T *)tv, n, thr, q.out_rows, q.out_cap, ctr);
+        tm.end();
         CCC_LAUNCH_CHECK("k_subprim");
     } else {
         // sub-primitive items and the narrow phase run over chunks of the pair list
@@ -1520,7 +1445,6 @@ static ccc_status run_query(const Query &q) {
         int64_t *items = ws.get_as<int64_t>(B * capM);
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
-        const unsigned gC = persistent_grid(q.ctx, chunk, B);
         const unsigned gS = persistent_grid(q.ctx, (chunk + kSubprimPairs - 1) / kSubprimPairs, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
         const unsigned gCcd = persistent_grid(q.ctx, capM, B, kCcdThreads);
@@ -1535,11 +1459,11 @@ static ccc_status run_query(const Query &q) {
                 const int64_t c1 = c0 + chunk;
                 tm.begin("subprim");
                 if (q.narrow == Narrow::PROXIMITY)
-                    hipLaunchKernelGGL((k_subprim<T, 1>), dim3(gC, B), dim3(kThreads), 0, s, pairs, capK, c0, c1, (const T *)q.t0,
-                                       (const T *)nullptr, n, thr, items, capM, ctr);
+                    hipLaunchKernelGGL((k_subprim<T, SUB_STATIC, kSubprimPairs>), dim3(gS, B), dim3(kThreads), 0, s, pairs, capK,
+                                       c0, c1, (const T *)q.t0, n, thr, items, capM, ctr);
                 else
-                    hipLaunchKernelGGL((k_subprim_moving<T, kSubprimPairs>), dim3(gS, B), dim3(kThreads), 0, s, pairs, capK, c0, c1,
-                                       (const T *)tv, n, items, capM, ctr);
+                    hipLaunchKernelGGL((k_subprim<T, SUB_MOVING, kSubprimPairs>), dim3(gS, B), dim3(kThreads), 0, s, pairs, capK,
+                                       c0, c1, (const T *)tv, n, thr, items, capM, ctr);
                 tm.end();
                 CCC_LAUNCH_CHECK("k_subprim");
                 tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
