@@ -785,7 +785,7 @@ struct SolveRec {
     double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
 };
 constexpr int kRootCap = 3 * kCcdThreads;
-constexpr int kCollectFlush = 256;
+constexpr int kCollectFlush = 64;
 
 __device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
     if (code == CUBIC_QUADRATIC) { r.v[0] = st.b; r.v[1] = st.c; r.v[2] = st.d; r.v[3] = 0; }
