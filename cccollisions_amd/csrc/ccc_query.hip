@@ -26,7 +26,7 @@ constexpr int kTravThreads = 128;
 // covers the trees built here (index radix trees, ceil(log2 N) + 1 deep) up to 2^22
 // leaves; a traversal that overflows it runs again with the deep one (trees up to 39
 // levels deep).
-constexpr int kStackShort = 16;
+constexpr int kStackShort = 24;   // up to three pushes per four-wide node (two binary levels)
 constexpr int kStackDeep = 32;
 constexpr int kStage = 512;       // staged pairs per wave before a flush
 constexpr int kFrontierDepth = 7; // queries start from the subtrees this deep below the root
@@ -244,6 +244,84 @@ __global__ void k_nodes(const T *__restrict__ bb_all, const int64_t *__restrict_
     nodes_all[(int64_t)b * (n - 1) + i] = nd;
 }
 
+// Four-wide view of the binary tree for the traversal.  For internal row i: the up to four
+// grandchildren of i, left to right (a child that is a leaf stands for itself), with their
+// TNode fp32 boxes as structure of arrays, refs and rightmost leaf positions; unused entries
+// (at the end) have a NaN box and rightmost INT_MIN.  Entry k > 0 starts at the leaf after
+// entry k-1's rightmost one (trees built here cover contiguous position ranges).  Testing a
+// grandchild's box without its parent's prunes nothing that matters when boxes contain their
+// children's (the traversal's assumption throughout: it starts below the root, and its node
+// tests are conservative fp32): a pair needs the leaf's exact box to overlap the query.  One
+// node is 128 bytes, one cache line.
+struct alignas(16) QNode {
+    float4 mnx, mny, mnz, mxx, mxy, mxz;
+    int4 ref;   // >= 0 internal row, < 0 leaf at position ~ref
+    int4 rm;    // rightmost leaf position; INT_MIN: unused entry
+};
+
+// side 0 / 1 of a TNode: box, ref and rightmost leaf of that child
+__device__ __forceinline__ void tnode_child(const TNode &nd, int side, float *mn, float *mx, int &ref, int &rm) {
+    if (side == 0) {
+        mn[0] = nd.a.x; mn[1] = nd.a.y; mn[2] = nd.a.z; mx[0] = nd.a.w; mx[1] = nd.b.x; mx[2] = nd.b.y;
+        ref = nd.d.x; rm = nd.d.z;
+    } else {
+        mn[0] = nd.b.z; mn[1] = nd.b.w; mn[2] = nd.c.x; mx[0] = nd.c.y; mx[1] = nd.c.z; mx[2] = nd.c.w;
+        ref = nd.d.y; rm = nd.d.w;
+    }
+}
+
+struct QEntry {
+    float mn[3], mx[3];
+    int ref, rm;
+};
+
+__global__ void k_qnodes(const TNode *__restrict__ nodes_all, int64_t n, QNode *__restrict__ q_all,
+                         const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (ctr[b].flags & DEV_BAD_TREE) return;
+    const TNode *nodes = nodes_all + (int64_t)b * (n - 1);
+    const TNode nd = nodes[i];
+    QEntry none;    // unused entry: a NaN box fails every comparison
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { none.mn[a] = NAN; none.mx[a] = NAN; }
+    none.ref = 0;
+    none.rm = INT_MIN;
+    // side s of row i -> its entries g[s][0], g[s][1] (g[s][1] unused when the child is a leaf)
+    QEntry g[2][2];
+    bool internal[2];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        QEntry c;
+        tnode_child(nd, side, c.mn, c.mx, c.ref, c.rm);
+        internal[side] = c.ref >= 0;
+        if (internal[side]) {
+            const TNode cn = nodes[c.ref];
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) tnode_child(cn, s2, g[side][s2].mn, g[side][s2].mx, g[side][s2].ref, g[side][s2].rm);
+        } else {
+            g[side][0] = c;
+            g[side][1] = none;
+        }
+    }
+    // left to right, unused entries last
+    const QEntry e0 = g[0][0];
+    const QEntry e1 = internal[0] ? g[0][1] : g[1][0];
+    const QEntry e2 = internal[0] ? g[1][0] : (internal[1] ? g[1][1] : none);
+    const QEntry e3 = internal[0] && internal[1] ? g[1][1] : none;
+    QNode q;
+    q.mnx = make_float4(e0.mn[0], e1.mn[0], e2.mn[0], e3.mn[0]);
+    q.mny = make_float4(e0.mn[1], e1.mn[1], e2.mn[1], e3.mn[1]);
+    q.mnz = make_float4(e0.mn[2], e1.mn[2], e2.mn[2], e3.mn[2]);
+    q.mxx = make_float4(e0.mx[0], e1.mx[0], e2.mx[0], e3.mx[0]);
+    q.mxy = make_float4(e0.mx[1], e1.mx[1], e2.mx[1], e3.mx[1]);
+    q.mxz = make_float4(e0.mx[2], e1.mx[2], e2.mx[2], e3.mx[2]);
+    q.ref = make_int4(e0.ref, e1.ref, e2.ref, e3.ref);
+    q.rm = make_int4(e0.rm, e1.rm, e2.rm, e3.rm);
+    q_all[(int64_t)b * (n - 1) + i] = q;
+}
+
 // Leaf record by position: the leaf box rounded inward to fp32 and the leaf's triangle id.
 // A query box that overlaps the inward box overlaps the exact box (fp rounding is monotone,
 // so fl(inward.max + thr) <= fl(max + thr) and fl(inward.min - thr) >= fl(min - thr)); only
@@ -340,10 +418,12 @@ __global__ __launch_bounds__(kFrontier) void k_frontier(const T *__restrict__ bb
 // collisions_static.cu:10-120 (ordering on) and collisions_continuous.cu:127-230,
 // :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
 // reported after the reference's exact test checkOverlap(query box, leaf box, threshold) on
-// the boxes stored in `bboxes`; the pair is (min id, max id).
+// the boxes stored in `bboxes`; the pair is (min id, max id).  The walk runs over the
+// four-wide nodes (QNode): each visit tests up to four grandchildren, which halves the
+// dependent node reads of a query against the binary nodes.
 template <typename T, bool PARTIAL, int DEPTH>
 __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__ bb_all, const int64_t *__restrict__ tree_all,
-                                                           const TNode *__restrict__ nodes_all,
+                                                           const QNode *__restrict__ qnodes_all,
                                                            const LeafRec *__restrict__ recs_all, int64_t n, T thr,
                                                            const uint8_t *__restrict__ to_check_all,
                                                            const int32_t *__restrict__ mesh_end,
@@ -352,18 +432,19 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     // Pending right children: with the short stack as 16-bit offsets from the start point's
     // node (a subtree's nodes lie close together in the trees built here; a farther node
     // ends the traversal like an overflow, and the call runs again with the deep stack).
-    using StackT = typename std::conditional<(DEPTH <= kStackShort), short, int>::type;
+    using StackT = typename std::conditional<(DEPTH < kStackDeep), short, int>::type;
+    constexpr int kPerIter = 4;    // pairs one iteration can add per lane: one per entry
     __shared__ StackT stack[DEPTH][kTravThreads];
-    __shared__ int stage[kTravThreads / 64][kStage + 128];                // the other triangle of each pair
-    __shared__ int lqid[kTravThreads / 64][64];                         // each lane's query triangle
-    __shared__ unsigned short stag[kTravThreads / 64][kStage + 128];   // lane << 10 | rank within the lane
+    __shared__ int stage[kTravThreads / 64][kStage + 64 * kPerIter];                // the other triangle of each pair
+    __shared__ int lqid[kTravThreads / 64][64];                                     // each lane's query triangle
+    __shared__ unsigned short stag[kTravThreads / 64][kStage + 64 * kPerIter];     // lane << 10 | rank within the lane
     __shared__ int loff[kTravThreads / 64][64];                         // a flush's offset of each lane
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
     const int64_t rows = 2 * n - 1;
     const T *bb = bb_all + (int64_t)b * rows * 6;
     const int64_t *tree = tree_all + (int64_t)b * rows * 4;
-    const TNode *nodes = nodes_all + (int64_t)b * (n - 1);
+    const QNode *qnodes = qnodes_all + (int64_t)b * (n - 1);
     const LeafRec *recs = recs_all + (int64_t)b * n;
     DevCounters *ctr = ctr_all + b;
     int2 *pairs = pairs_all + (int64_t)b * cap;
@@ -428,57 +509,73 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
         }
         mine = 0;
     };
+    // a pending subtree onto the stack; an overflow (or, with 16-bit entries, a node too far
+    // from the start node) ends this lane and flags the call for the deep stack
+    auto push = [&](int ref) {
+        const int v = sizeof(StackT) < sizeof(int) ? ref - sbase : ref;
+        const bool fits = sizeof(StackT) == sizeof(int) || (v >= -32768 && v <= 32767);
+        if (sp >= DEPTH || !fits) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
+        else stack[sp++][tid] = (StackT)v;
+    };
+    auto leaf_pair = [&](int ref) -> int {     // the pair (query, leaf ~ref) or -1
+        const LeafRec lr = recs[~ref];
+        if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~ref))), thr)) {
+            const int oid = lr.id;
+            if (!PARTIAL || !to_check[oid] || qid < oid) return oid;
+        }
+        return -1;
+    };
     while (__any(alive)) {
-        int oid0 = -1, oid1 = -1;
-        bool ol = false, orr = false;
-        TNode nd;
+        int o0 = -1, o1 = -1, o2 = -1, o3 = -1;     // pairs found in this iteration
+        int r0 = 0, r1 = 0, r2 = 0, r3 = 0;         // children (entries) that may be descended into
+        unsigned im = 0;                            // bit k: descend into r_k
         if (alive) {
-            if (node >= 0) {
-                nd = nodes[node];
-            } else {
-                // a start point that is a leaf: examined once as a left child that passed its tests
-                nd.a = make_float4(-INFINITY, -INFINITY, -INFINITY, INFINITY);
-                nd.b = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
-                nd.c = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
-                nd.d = make_int4(node, 0, ~node, INT_MIN);
-            }
-            ol = qx0 <= nd.a.w && qx1 >= nd.a.x && qy0 <= nd.b.x && qy1 >= nd.a.y && qz0 <= nd.b.y && qz1 >= nd.a.z;
-            orr = qx0 <= nd.c.y && qx1 >= nd.b.z && qy0 <= nd.c.z && qy1 >= nd.b.w && qz0 <= nd.c.w && qz1 >= nd.c.x;
-            if (!PARTIAL) {
-                ol = ol && nd.d.z > (int)p;
-                orr = orr && nd.d.w > (int)p;
-            }
-            // the right child's first leaf follows the left child's last one (trees built
-            // here cover contiguous position ranges); a node is only entered when its first
-            // leaf is below lim, so the left child never needs the test
-            orr = orr && nd.d.z + 1 < lim;
-            if (ol && nd.d.x < 0) {
-                const LeafRec lr = recs[~nd.d.x];
-                if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~nd.d.x))), thr)) {
-                    const int oid = lr.id;
-                    if (!PARTIAL || !to_check[oid] || qid < oid) oid0 = oid;
+            {
+                QNode qn;
+                if (node >= 0) {
+                    qn = qnodes[node];
+                } else {
+                    // a start point that is a leaf: entry 0 is the leaf, with an unbounded box
+                    qn.mnx = make_float4(-INFINITY, NAN, NAN, NAN);
+                    qn.mxx = make_float4(INFINITY, NAN, NAN, NAN);
+                    qn.mny = qn.mnx; qn.mnz = qn.mnx; qn.mxy = qn.mxx; qn.mxz = qn.mxx;
+                    qn.ref = make_int4(node, 0, 0, 0);
+                    qn.rm = make_int4(~node, INT_MIN, INT_MIN, INT_MIN);
                 }
-            }
-            if (orr && nd.d.y < 0) {
-                const LeafRec lr = recs[~nd.d.y];
-                if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~nd.d.y))), thr)) {
-                    const int oid = lr.id;
-                    if (!PARTIAL || !to_check[oid] || qid < oid) oid1 = oid;
+                const float mnx[4] = {qn.mnx.x, qn.mnx.y, qn.mnx.z, qn.mnx.w}, mxx[4] = {qn.mxx.x, qn.mxx.y, qn.mxx.z, qn.mxx.w};
+                const float mny[4] = {qn.mny.x, qn.mny.y, qn.mny.z, qn.mny.w}, mxy[4] = {qn.mxy.x, qn.mxy.y, qn.mxy.z, qn.mxy.w};
+                const float mnz[4] = {qn.mnz.x, qn.mnz.y, qn.mnz.z, qn.mnz.w}, mxz[4] = {qn.mxz.x, qn.mxz.y, qn.mxz.z, qn.mxz.w};
+                const int rf[4] = {qn.ref.x, qn.ref.y, qn.ref.z, qn.ref.w}, rmv[4] = {qn.rm.x, qn.rm.y, qn.rm.z, qn.rm.w};
+                int oid[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    bool hit = qx0 <= mxx[k] && qx1 >= mnx[k] && qy0 <= mxy[k] && qy1 >= mny[k] && qz0 <= mxz[k] && qz1 >= mnz[k];
+                    if (!PARTIAL) hit = hit && rmv[k] > (int)p;
+                    // entry k starts after entry k-1's last leaf; the node itself (so entry 0)
+                    // is only entered when its first leaf is below lim
+                    if (k > 0) hit = hit && rmv[k - 1] + 1 < lim;
+                    oid[k] = -1;
+                    if (hit && rf[k] < 0) oid[k] = leaf_pair(rf[k]);
+                    else if (hit) im |= 1u << k;
                 }
+                o0 = oid[0]; o1 = oid[1]; o2 = oid[2]; o3 = oid[3];
+                r0 = rf[0]; r1 = rf[1]; r2 = rf[2]; r3 = rf[3];
             }
         }
         // stage this iteration's pairs in the wave's LDS buffer (tagged with lane and rank)
         {
-            const unsigned c = (oid0 >= 0 ? 1u : 0u) + (oid1 >= 0 ? 1u : 0u);
-            const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u);
-            const unsigned below = lanes_below(b0) + 2u * lanes_below(b1);
-            const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1);
+            const unsigned c = (o0 >= 0 ? 1u : 0u) + (o1 >= 0 ? 1u : 0u) + (o2 >= 0 ? 1u : 0u) + (o3 >= 0 ? 1u : 0u);
+            const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+            const unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
+            const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1) + 4u * (unsigned)__popcll(b2);
             if (tot) {
                 const int f = fill[w];
                 int pos = f + (int)below;
                 const unsigned lane_tag = (unsigned)lane << 10;
-                if (oid0 >= 0) { stage[w][pos] = oid0; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
-                if (oid1 >= 0) { stage[w][pos] = oid1; stag[w][pos] = (unsigned short)(lane_tag | mine++); }
+                if (o0 >= 0) { stage[w][pos] = o0; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
+                if (o1 >= 0) { stage[w][pos] = o1; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
+                if (o2 >= 0) { stage[w][pos] = o2; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
+                if (o3 >= 0) { stage[w][pos] = o3; stag[w][pos] = (unsigned short)(lane_tag | mine++); }
                 const int nf = f + (int)tot;
                 if (nf >= kStage) {
                     flush_stage(nf);
@@ -489,18 +586,17 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             }
         }
         if (alive) {
-            const bool tl = ol && nd.d.x >= 0, tr = orr && nd.d.y >= 0;
-            if (!tl && !tr) {
+            if (im == 0) {
                 if (sp == 0) alive = false;
                 else node = sizeof(StackT) < sizeof(int) ? sbase + (int)stack[--sp][tid] : (int)stack[--sp][tid];
             } else {
-                node = tl ? nd.d.x : nd.d.y;
-                if (tl && tr) {
-                    const int push = sizeof(StackT) < sizeof(int) ? nd.d.y - sbase : nd.d.y;
-                    const bool fits = sizeof(StackT) == sizeof(int) || (push >= -32768 && push <= 32767);
-                    if (sp >= DEPTH || !fits) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
-                    else stack[sp++][tid] = (StackT)push;
-                }
+                // descend into the leftmost; the others wait on the stack, right to left, so
+                // that the next one to the right comes off first
+                const int lo = __ffs(im) - 1;
+                node = lo == 0 ? r0 : lo == 1 ? r1 : lo == 2 ? r2 : r3;
+                if ((im & 8u) && lo != 3) push(r3);
+                if (alive && (im & 4u) && lo != 2) push(r2);
+                if (alive && (im & 2u) && lo != 1) push(r1);
             }
         }
     }
@@ -1301,6 +1397,7 @@ static ccc_status run_query(const Query &q) {
     int *rm = ws.get_as<int>(B * nint);
     int *lo = ws.get_as<int>(B * nint);
     TNode *nodes = ws.get_as<TNode>(B * nint);
+    QNode *qnodes = ws.get_as<QNode>(B * nint);
     LeafRec *recs = ws.get_as<LeafRec>(B * n);
     Frontier *frontier = ws.get_as<Frontier>(B);
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
@@ -1346,6 +1443,7 @@ static ccc_status run_query(const Query &q) {
     }
     hipLaunchKernelGGL(k_rightmost, dim3(grid_for(n, 256), B), dim3(256), 0, s, q.tree, n, climb, rm, ctr);
     if (n > 1) hipLaunchKernelGGL((k_nodes<T>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, bb, q.tree, rm, n, q.thr, nodes, ctr);
+    if (n > 1) hipLaunchKernelGGL(k_qnodes, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, nodes, n, qnodes, ctr);
     hipLaunchKernelGGL((k_leaf_recs<T>), dim3(grid_for(n, 256), B), dim3(256), 0, s, bb, q.tree, n, recs, ctr);
     hipLaunchKernelGGL((k_frontier<T>), dim3(B), dim3(kFrontier), 0, s, bb, q.tree, rm, lo, n, q.thr, frontier, ctr);
     tm.end();
@@ -1364,7 +1462,7 @@ static ccc_status run_query(const Query &q) {
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         tm.begin("traverse");
 #define CCC_TRAVERSE(P, D)                                                                                   \
-    hipLaunchKernelGGL((k_traverse<T, P, D>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree, nodes, recs, n, \
+    hipLaunchKernelGGL((k_traverse<T, P, D>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree, qnodes, recs, n, \
                        (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier)
         if (q.to_check) {
             if (deep) CCC_TRAVERSE(true, kStackDeep);

@@ -527,13 +527,16 @@ def _cluster(n, seed):
     return t0, t1
 
 
-def test_deep_tree_reruns_with_the_deep_stack(ccc, oracle):
-    """A spine 24 levels deep overflows the short traversal stack below the frontier; the
-    traversal runs again with the deep stack and finds every pair the oracle finds."""
-    t0, t1 = _cluster(50, seed=21)
-    bb, tree = _left_spine_tree(t0, t1, 24)
+@pytest.mark.parametrize("depth", [24, 60])
+def test_deep_tree_reruns_with_the_deep_stack(ccc, oracle, depth):
+    """A left spine keeps one subtree pending per two levels of the four-wide traversal: 24
+    levels fit the short stack (24 entries), 60 overflow it below the frontier, and the
+    traversal runs again with the deep stack.  Both find every pair the oracle finds."""
+    n = 2 * depth + 2
+    t0, t1 = _cluster(n, seed=21)
+    bb, tree = _left_spine_tree(t0, t1, depth)
     pairs, total = oracle.traverse(bb, tree, 0.0)
-    assert total == 50 * 49 // 2
+    assert total == n * (n - 1) // 2
     np.testing.assert_array_equal(sorted_rows(pairs), sorted_rows(oracle.bruteforce_pairs(bb, tree, 0.0)))
     coll, roots, counts = oracle.ccd(t0, t1, pairs)
     gc, gr = ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
@@ -542,11 +545,11 @@ def test_deep_tree_reruns_with_the_deep_stack(ccc, oracle):
 
 
 def test_tree_deeper_than_the_deep_stack_raises(ccc, oracle):
-    """Beyond the deep stack (a 40-level spine: 33 pending subtrees below the start points) the
-    call fails with an error instead of returning a partial pair list."""
-    t0, t1 = _cluster(82, seed=22)
-    bb, tree = _left_spine_tree(t0, t1, 40)
-    assert oracle.traverse(bb, tree, 0.0)[1] == 82 * 81 // 2     # the oracle's stack is deeper
+    """Beyond the deep stack (an 80-level spine: 36 pending subtrees below the start points
+    with four-wide nodes, more than the 32 entries) the call fails with an error instead of
+    returning a partial pair list."""
+    t0, t1 = _cluster(162, seed=22)
+    bb, tree = _left_spine_tree(t0, t1, 80)
     with pytest.raises(RuntimeError, match="deeper than the traversal stack"):
         ccc.find_collisions_continuous(_t(bb[None]), _t(tree[None]), _t(t0[None]), _t(t1[None]), 64, 64)
 
