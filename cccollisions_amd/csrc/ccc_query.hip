@@ -433,11 +433,11 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     // node (a subtree's nodes lie close together in the trees built here; a farther node
     // ends the traversal like an overflow, and the call runs again with the deep stack).
     using StackT = typename std::conditional<(DEPTH < kStackDeep), short, int>::type;
-    constexpr int kPerIter = 4;    // pairs one iteration can add per lane: one per entry
     __shared__ StackT stack[DEPTH][kTravThreads];
-    __shared__ int stage[kTravThreads / 64][kStage + 64 * kPerIter];                // the other triangle of each pair
+    constexpr int kStageCap = kStage + 128;   // an iteration that would overflow it flushes first
+    __shared__ int stage[kTravThreads / 64][kStageCap];                             // the other triangle of each pair
     __shared__ int lqid[kTravThreads / 64][64];                                     // each lane's query triangle
-    __shared__ unsigned short stag[kTravThreads / 64][kStage + 64 * kPerIter];     // lane << 10 | rank within the lane
+    __shared__ unsigned short stag[kTravThreads / 64][kStageCap];                  // lane << 10 | rank within the lane
     __shared__ int loff[kTravThreads / 64][64];                         // a flush's offset of each lane
     __shared__ int fill[kTravThreads / 64];
     const int b = blockIdx.y, tid = threadIdx.x, w = tid >> 6;
@@ -569,7 +569,11 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
             const unsigned below = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
             const unsigned tot = (unsigned)__popcll(b0) + 2u * (unsigned)__popcll(b1) + 4u * (unsigned)__popcll(b2);
             if (tot) {
-                const int f = fill[w];
+                int f = fill[w];
+                if (f + (int)tot > kStageCap) {     // would not fit: write out what is staged first
+                    flush_stage(f);
+                    f = 0;
+                }
                 int pos = f + (int)below;
                 const unsigned lane_tag = (unsigned)lane << 10;
                 if (o0 >= 0) { stage[w][pos] = o0; stag[w][pos++] = (unsigned short)(lane_tag | mine++); }
