@@ -740,10 +740,13 @@ __global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restri
 }
 
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
-// Threads (items) per k_ccd tile.  Two waves per block rather than four: the block phases
-// wait at barriers for their slowest wave, and 8 blocks of 2 waves per CU keep the SIMDs
-// busier than 4 of 4 (k_ccd 206 -> 193 ms per step on the 130k cloth).
-constexpr int kCcdThreads = 128;
+// Threads (items) per k_ccd tile: one wave per block.  The block phases wait at barriers for
+// their slowest wave, so smaller blocks waste less of that wait (k_ccd per step on the 130k
+// cloth: 4 waves per block 206 ms, 2 waves 193 then 175, 1 wave 172.5).
+constexpr int kCcdThreads = 64;
+// The collect mode keeps two-wave tiles: its output stage needs fewer registers than the
+// scatter, and one-wave tiles measured slower there (20.0 against 18.3 ms per pass).
+template <int MODE> constexpr int ccd_threads() { return MODE == MODE_IMPULSE ? kCcdThreads : 2 * kCcdThreads; }
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
 // tiles and a tile adds at most 3 * kCcdThreads.
 constexpr int kFlush = kCcdThreads / 2;
@@ -874,17 +877,17 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //      table.  The scatter stays in this kernel: run as a kernel of its own over
 //      per-workgroup collision lists it took 14 ms per pass on the 130k cloth against ~6
 //      here, where other workgroups' fp64 work hides its atomic latency.
-// Each thread's item of the next tile is loaded while this tile is solved.  Occupancy: the
-// collect mode runs five waves per SIMD (<= 96 VGPRs, 10 KB of LDS per block), the impulse
-// mode four (128 VGPRs, 17 KB: eight blocks per CU).  At five waves the impulse mode spills
-// about a hundred registers and ran 40% slower; at three 13% slower.  Staging each wave's
+// Each thread's item of the next tile is loaded while this tile is solved.  Occupancy (one-
+// wave blocks): the collect mode 90 VGPRs and 9.3 KB of LDS per block (17 blocks per CU),
+// the impulse mode four waves per SIMD (128 VGPRs, 8.7 KB: sixteen blocks per CU).  At five
+// waves the impulse mode spills about a hundred registers and ran 40% slower; at three 13%
+// slower.  Staging each wave's
 // query triangle in LDS (so that half the corner reads leave the vector memory path) made
 // the impulse mode spill and took 196-209 ms per step instead of 177.
 struct SolveRec {
     int64_t item;
     double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
 };
-constexpr int kRootCap = 3 * kCcdThreads;
 constexpr int kCollectFlush = 64;
 
 __device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
@@ -902,26 +905,29 @@ __device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   DevCounters *__restrict__ ctr_all) {
+    constexpr int NT = ccd_threads<MODE>();     // items per tile, threads per block
+    constexpr int kRootCapT = 3 * NT;
+    static_assert(MODE != MODE_IMPULSE || NT == kCcdThreads, "the impulse table is walked by kCcdThreads threads");
     // the records (phases A-C) and the root list (phases B-C) share their LDS with the
     // impulse table of phase D
-    constexpr size_t kSolveBytes = sizeof(SolveRec) * kCcdThreads + (sizeof(T) + sizeof(unsigned short)) * kRootCap;
+    constexpr size_t kSolveBytes = sizeof(SolveRec) * NT + (sizeof(T) + sizeof(unsigned short)) * kRootCapT;
     constexpr size_t kScatterBytes = ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush;
     __shared__ __align__(16) unsigned char s_raw[kSolveBytes > kScatterBytes ? kSolveBytes : kScatterBytes];
     SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
-    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * kCcdThreads);
-    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * kCcdThreads + sizeof(T) * kRootCap);
+    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * NT);
+    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * NT + sizeof(T) * kRootCapT);
     __shared__ int s_cnt[5];
     __shared__ int s_nroot;
     // accepted collisions wait here: impulse mode until kFlush of them are scattered,
     // collect mode until kCollectFlush of them are written as output rows with one
     // reservation (the row counter is one contended device address)
-    constexpr int kStageCap = MODE == MODE_IMPULSE ? kImpStage : kCollectFlush + 3 * kCcdThreads;
+    constexpr int kStageCap = MODE == MODE_IMPULSE ? kImpStage : kCollectFlush + 3 * NT;
     __shared__ int64_t s_item[kStageCap];
     __shared__ T s_root[kStageCap];
     __shared__ unsigned long long s_obase;
@@ -945,7 +951,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     // dx / cnt with one set of global atomics.
     static_assert(ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush <= sizeof(s_raw),
                   "the impulse table and parameters fit the phase A-C buffers");
-    static_assert(kFlush <= kCcdThreads && (4 * kFlush) % kCcdThreads == 0, "whole rounds of corners");
+    static_assert(kFlush <= NT && (4 * kFlush) % NT == 0, "whole rounds of corners");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
     ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + ImpTable<T>::bytes());
     // Scatters staged collisions [0, f) in chunks of kFlush: one thread per collision
@@ -974,7 +980,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
                 s_imp[tid] = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
             }
             __syncthreads();
-            for (int t = tid; t < 4 * kFlush; t += kCcdThreads) {
+            for (int t = tid; t < 4 * kFlush; t += NT) {
                 const int k = t / kFlush, jc = t % kFlush;
                 if (jc < nf) {
                     int64_t x, y;
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
         __syncthreads();
         const unsigned long long ob = s_obase;
         int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
-        for (int j = tid; j < f; j += kCcdThreads) {
+        for (int j = tid; j < f; j += NT) {
             const unsigned long long slot = ob + (unsigned long long)j;
             if (slot < (unsigned long long)out_cap) {
                 int64_t x, y;
@@ -1018,11 +1024,11 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
     // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
-    const int64_t ntiles = (M + kCcdThreads - 1) / kCcdThreads;
+    const int64_t ntiles = (M + NT - 1) / NT;
     const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
     const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
     auto item_of = [&](int64_t k) -> int64_t {
-        const int64_t i = (tbase + k) * kCcdThreads + tid;
+        const int64_t i = (tbase + k) * NT + tid;
         return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
     };
     int64_t next_item = item_of(blockIdx.x / kXcds);
@@ -1078,7 +1084,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
         __syncthreads();
         // ---- C: filter one root record per thread
         const int nr = s_nroot;
-        for (int base = 0; base < nr; base += kCcdThreads) {
+        for (int base = 0; base < nr; base += NT) {
             const int j = base + tid;
             bool ok = false;
             int64_t rit = 0;
@@ -1549,7 +1555,8 @@ static ccc_status run_query(const Query &q) {
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         const unsigned gS = persistent_grid(q.ctx, (chunk + kSubprimPairs - 1) / kSubprimPairs, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
-        const unsigned gCcd = persistent_grid(q.ctx, capM, B, kCcdThreads);
+        const unsigned gCcdC = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_COLLECT>());
+        const unsigned gCcdI = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_IMPULSE>());
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         if (ccd) {
             hipLaunchKernelGGL((k_vertex_records<T>), dim3(grid_for(3 * n, 256), B), dim3(256), 0, s, (const T *)q.t0,
@@ -1573,11 +1580,11 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
                                        q.out_rows, q.out_cap, ctr);
                 else if (q.narrow == Narrow::CCD_COLLECT)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gCcd, B), dim3(kCcdThreads), 0, s, items, capM, (const T *)tv,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gCcdC, B), dim3(ccd_threads<MODE_COLLECT>()), 0, s, items, capM, (const T *)tv,
                                        (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
                                        (T *)nullptr, (T *)nullptr, (int *)nullptr, ctr);
                 else
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gCcd, B), dim3(kCcdThreads), 0, s, items, capM, (const T *)tv,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM, (const T *)tv,
                                        (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
                                        (T *)q.dv, (T *)q.dx, q.cnt, ctr);
                 tm.end();
