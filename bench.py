@@ -44,6 +44,9 @@ def parse():
                     help="also time the packed multi-mesh path against the plain per-mesh path on this many "
                          "folded cloths of 2k-200k triangles (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=1)
+    ap.add_argument("--no-collect", dest="collect", action="store_false",
+                    help="skip timing find_collisions_continuous (collect mode) on the cloth")
+    ap.add_argument("--collect-rows", type=int, default=288, help="collision rows per triangle for the collect timing")
     ap.add_argument("--no-garment", dest="garment", action="store_false",
                     help="skip the garment-on-body scene (static 24k-triangle capsule body + 100k-triangle "
                          "sleeve moving inward)")
@@ -56,13 +59,13 @@ def parse():
     ap.add_argument("--serving-load", type=float, default=0.5,
                     help="offered load of the open-loop run as a fraction of the saturation rate")
     ap.add_argument("--main-only", action="store_true",
-                    help="time only the 130k-triangle cloth (no garment, batched or serving section)")
+                    help="time only the 130k-triangle cloth (no collect, garment, batched or serving section)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy dispatch of "
                          "known size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.main_only:
-        args.garment, args.batched_sims, args.serving_sims = False, 0, 0
+        args.garment, args.collect, args.batched_sims, args.serving_sims = False, False, 0, 0
     return args
 
 
@@ -178,6 +181,32 @@ def garment_section(ccc, torch, scenes, args):
             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
             "first_pass_counts": list(map(int, step.counts[0])),
             "mean_collisions_per_iter": float(np.mean([c[3] for c in timed]))}
+
+
+def collect_section(ccc, torch, step, args):
+    """find_collisions_continuous (collect mode: collision rows and roots instead of impulses)
+    on the cloth's first-pass inputs, with room for every collision."""
+    t1 = step.x1[step.faces].unsqueeze(0).contiguous()
+    bb, tree = ccc.bvh_motion(step.t0, t1)
+
+    def call():
+        return ccc.find_collisions_continuous(bb, tree, step.t0, t1, args.max_candidates, args.collect_rows)
+
+    call()
+    torch.cuda.synchronize()
+    ccc.set_profiling(True)
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - a)
+    kt = ccc.kernel_times()
+    ccc.set_profiling(False)
+    counts = ccc.last_counts()[0]
+    return {"calls": 3, "ms_per_call": 1e3 * float(np.median(ts)), "ccd_collect_ms_per_call": kt["ccd_collect"][0] / 3,
+            "rows_capacity": step.t0.shape[1] * args.collect_rows, "counts": list(map(int, counts))}
 
 
 def batched_section(ccc, torch, scenes, args):
@@ -379,6 +408,8 @@ def main():
         "roofline": roof,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items())},
     }
+    if args.collect:
+        out["collect"] = collect_section(ccc, torch, step, args)
     if args.garment:
         out["garment"] = garment_section(ccc, torch, scenes, args)
     if args.batched_sims:
