@@ -343,8 +343,8 @@ def cpu_baseline(scene, stride, iters):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the workload shards by simulation; see the design notes)", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
+              "(across GPUs the workload would shard by simulation, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
     import cccollisions_amd as ccc
