@@ -413,6 +413,39 @@ __global__ __launch_bounds__(kFrontier) void k_frontier(const T *__restrict__ bb
     }
 }
 
+// fp32 box of each block of kTravThreads query leaves (the union of the lanes' outward-
+// rounded query boxes; empty past the last leaf), so that a traversal block whose queries
+// cannot enter its start point ends before it starts.
+template <typename T>
+__global__ __launch_bounds__(kTravThreads) void k_qblock_boxes(const T *__restrict__ bb_all, int64_t n, int nqb,
+                                                               float *__restrict__ qbox_all, const DevCounters *__restrict__ ctr) {
+    const int b = blockIdx.y;
+    const int64_t p = (int64_t)blockIdx.x * kTravThreads + threadIdx.x;
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (p < n && !(ctr[b].flags & DEV_BAD_TREE)) {
+        const T *q = bb_all + (int64_t)b * (2 * n - 1) * 6 + 6 * (n - 1 + p);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] = f32_down((double)q[k]);
+            v[k + 3] = f32_up((double)q[k + 3]);
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] = fminf(v[k], __shfl_xor(v[k], d, 64));
+            v[k + 3] = fmaxf(v[k + 3], __shfl_xor(v[k + 3], d, 64));
+        }
+    }
+    static_assert(kTravThreads == 64, "one wave per query block");
+    if (threadIdx.x == 0) {
+        float *o = qbox_all + ((int64_t)b * nqb + blockIdx.x) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = v[k];
+    }
+}
+
 // ---- traversal ----------------------------------------------------------------------------
 // collisions_static.cu:10-120 (ordering on) and collisions_continuous.cu:127-230,
 // :1079-1100 (PARTIAL: ordering off, queries with to_check[position]).  A leaf child is
@@ -427,7 +460,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                                                            const uint8_t *__restrict__ to_check_all,
                                                            const int32_t *__restrict__ mesh_end,
                                                            int2 *__restrict__ pairs_all, int64_t cap,
-                                                           DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all) {
+                                                           DevCounters *__restrict__ ctr_all, const Frontier *__restrict__ fr_all,
+                                                           const float *__restrict__ qbox_all) {
     // Pending right children: with the short stack as 16-bit offsets from the start point's
     // node (a subtree's nodes lie close together in the trees built here; a farther node
     // ends the traversal like an overflow, and the call runs again with the deep stack).
@@ -447,8 +481,6 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     const LeafRec *recs = recs_all + (int64_t)b * n;
     DevCounters *ctr = ctr_all + b;
     int2 *pairs = pairs_all + (int64_t)b * cap;
-    if ((tid & 63) == 0) fill[w] = 0;
-    __syncthreads();
 
     // block -> (start point s, query block); query blocks in XCD-contiguous order (the
     // query-block count nqb is a multiple of kXcds)
@@ -457,6 +489,20 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
     const int64_t qblock = (int64_t)(local % kXcds) * (nqb / kXcds) + local / kXcds;
     const int64_t p = qblock * blockDim.x + tid;
     const Frontier &fr = fr_all[b];
+    {
+        // the whole block ends here when none of its queries can enter the start point (the
+        // same box and rightmost tests as each lane's below, on the block's union box)
+        const float *qb = qbox_all + ((int64_t)b * nqb + qblock) * 6;
+        bool dead = s >= fr.count;
+        if (!dead) {
+            const float *fb = fr.box[s];
+            dead = !(qb[0] <= fb[3] && qb[3] >= fb[0] && qb[1] <= fb[4] && qb[4] >= fb[1] && qb[2] <= fb[5] && qb[5] >= fb[2]);
+            if (!PARTIAL) dead = dead || fr.rightmost[s] <= (int)(qblock * kTravThreads);
+        }
+        if (dead) return;
+    }
+    if ((tid & 63) == 0) fill[w] = 0;
+    __syncthreads();
     bool active = p < n && n > 1 && !(ctr->flags & DEV_BAD_TREE) && s < fr.count;
     if (PARTIAL && active) active = to_check_all[(int64_t)b * n + p] != 0;
     const uint8_t *to_check = PARTIAL ? to_check_all + (int64_t)b * n : nullptr;
@@ -1465,6 +1511,11 @@ static ccc_status run_query(const Query &q) {
     int2 *pairs = nullptr;
     // one block per (start point, query block); query blocks rounded to a multiple of kXcds
     const unsigned gT = kFrontier * (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
+    const int nqb = (int)(gT / kFrontier);
+    float *qbox = ws.get_as<float>(B * (int64_t)nqb * 6);
+    if (ws.failed()) return CCC_ERR_NO_MEMORY;
+    hipLaunchKernelGGL((k_qblock_boxes<T>), dim3(nqb, B), dim3(kTravThreads), 0, s, bb, n, nqb, qbox, ctr);
+    CCC_LAUNCH_CHECK("k_qblock_boxes");
     bool deep = false;   // the short stack overflowed once: run with the deep one
     for (int attempt = 0; attempt < 2; ++attempt) {
         pairs = ws.get_as<int2>(B * capK);
@@ -1472,7 +1523,7 @@ static ccc_status run_query(const Query &q) {
         tm.begin("traverse");
 #define CCC_TRAVERSE(P, D)                                                                                   \
     hipLaunchKernelGGL((k_traverse<T, P, D>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree, qnodes, recs, n, \
-                       (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier)
+                       (T)q.thr, q.to_check, q.mesh_end, pairs, capK, ctr, frontier, qbox)
         if (q.to_check) {
             if (deep) CCC_TRAVERSE(true, kStackDeep);
             else CCC_TRAVERSE(true, kStackShort);
