@@ -35,6 +35,7 @@ compute_impulses_partial = _ext.compute_impulses_partial
 compute_impulses_from_collisions = _ext.compute_impulses_from_collisions
 compute_impulses_packed = _ext.compute_impulses_packed
 set_fixed_capacity = _ext.set_fixed_capacity
+set_pair_chunk = _ext.set_pair_chunk
 last_counts = _ext.last_counts
 set_profiling = _ext.set_profiling
 kernel_times = _ext.kernel_times
@@ -50,4 +51,5 @@ ENTRY_POINTS = (
 # beyond the reference's API: many meshes of different sizes in one pass of launches
 EXTENSIONS = ("compute_impulses_packed",)
 
-__all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "last_counts", "set_profiling", "kernel_times"]
+__all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "set_pair_chunk", "last_counts", "set_profiling",
+                                                    "kernel_times"]

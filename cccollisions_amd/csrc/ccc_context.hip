@@ -18,6 +18,16 @@ Workspace::~Workspace() {
     }
 }
 
+void Workspace::release(void *p) {
+    for (size_t i = 0; i < ptrs_.size(); ++i) {
+        if (ptrs_[i] != p) continue;
+        if (ctx_ && ctx_->free_fn) ctx_->free_fn(ctx_->user, p, (void *)s_);
+        else (void)hipFreeAsync(p, s_);
+        ptrs_.erase(ptrs_.begin() + (std::ptrdiff_t)i);
+        return;
+    }
+}
+
 void *Workspace::get(size_t bytes) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~(size_t)255;
@@ -106,6 +116,13 @@ const char *ccc_status_string(ccc_status status) {
 const char *ccc_last_error(void) { return ccc::g_last_error.c_str(); }
 
 int ccc_abi_version(void) { return CCC_ABI_VERSION; }
+
+ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs) {
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    if (pairs < 0) return ccc::bad_arg("pair chunk must be >= 0");
+    ctx->pair_chunk = pairs;
+    return CCC_OK;
+}
 
 ccc_status ccc_context_set_profiling(ccc_context *ctx, int on) {
     if (!ctx) return ccc::bad_arg("context is NULL");

@@ -301,12 +301,16 @@ __device__ __forceinline__ double dd_cube_lead(double b) {
     return q + t;
 }
 
-// collisions_continuous.cu:448-464 add_root (window [-1e-16, 1+1e-16], slots x,y,z).
+// collisions_continuous.cu:448-464 add_root (window [-1e-16, 1+1e-16], slots x,y,z).  The
+// reference's parameter is `T root`: the double root is rounded to T at the call and the
+// window is tested on the rounded value (for float, a root in (1, 1 + 2^-24] becomes 1.0f and
+// is kept).
 template <typename T> __device__ __forceinline__ void add_root(T *slots, double root, int &n) {
-    if (root < -1e-16 || root > 1 + 1e-16) return;
-    if (n == 0) slots[0] = (T)root;
-    else if (n == 1) slots[1] = (T)root;
-    else if (n == 2) slots[2] = (T)root;
+    const T r = (T)root;
+    if ((double)r < -1e-16 || (double)r > 1 + 1e-16) return;
+    if (n == 0) slots[0] = r;
+    else if (n == 1) slots[1] = r;
+    else if (n == 2) slots[2] = r;
     n++;
 }
 

@@ -18,6 +18,8 @@ struct ccc_context {
     ccc_free_fn free_fn;
     void *user;
     int num_cus;
+    // pairs per narrow-phase chunk over the whole batch (0: the default, 2^25)
+    int64_t pair_chunk = 0;
     // opt-in kernel timing (HIP events on the call's stream), accumulated per kernel name
     bool profiling = false;
     std::mutex prof_mu;
@@ -38,6 +40,8 @@ class Workspace {
     ~Workspace();
     void *get(size_t bytes);
     template <typename P> P *get_as(size_t count) { return static_cast<P *>(get(count * sizeof(P))); }
+    // Returns a buffer obtained from get() before the Workspace ends (stream-ordered).
+    void release(void *p);
     bool failed() const { return failed_; }
 
   private:

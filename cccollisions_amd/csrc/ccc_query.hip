@@ -1420,8 +1420,9 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
     }
 }
 
-// Pairs per narrow-phase chunk: the item list holds at most 15 items per pair of a chunk,
-// 4 GiB per batch element at this size.
+// Pairs per narrow-phase chunk over all batch elements of a call (the context may set
+// another size): the item list holds at most 15 items per pair of a chunk, 4 GiB at this
+// size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
 constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim
 
@@ -1518,6 +1519,7 @@ static ccc_status run_query(const Query &q) {
     CCC_LAUNCH_CHECK("k_qblock_boxes");
     bool deep = false;   // the short stack overflowed once: run with the deep one
     for (int attempt = 0; attempt < 2; ++attempt) {
+        if (pairs) ws.release(pairs);     // a re-run: the first attempt's pairs are not kept
         pairs = ws.get_as<int2>(B * capK);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         tm.begin("traverse");
@@ -1597,8 +1599,11 @@ static ccc_status run_query(const Query &q) {
         tm.end();
         CCC_LAUNCH_CHECK("k_subprim");
     } else {
-        // sub-primitive items and the narrow phase run over chunks of the pair list
-        const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, kPairChunk), 1);
+        // sub-primitive items and the narrow phase run over chunks of the pair list; the
+        // chunk bounds the pairs of all batch elements together (the item buffer holds B
+        // chunks), so the workspace does not grow with the batch
+        const int64_t budget = q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
+        const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, budget / B), 1);
         const int64_t capM = 15 * chunk;
         int64_t *items = ws.get_as<int64_t>(B * capM);
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;

@@ -424,6 +424,14 @@ std::vector<torch::Tensor> compute_impulses_from_collisions(at::Tensor collision
 // ---- extras (not in the reference) -------------------------------------------------------
 void set_fixed_capacity(bool on) { g_flags = on ? (g_flags | CCC_FLAG_FIXED_CAPACITY) : (g_flags & ~CCC_FLAG_FIXED_CAPACITY); }
 
+// Pairs per narrow-phase chunk (0: default) of the current device's context.
+void set_pair_chunk(int64_t pairs) {
+    const int dev = c10::hip::current_device();
+    auto t = at::empty({0}, at::TensorOptions().device(at::Device(at::kCUDA, dev)));
+    ccc_status st = ccc_context_set_pair_chunk(context_for(t), pairs);
+    TORCH_CHECK(st == CCC_OK, "set_pair_chunk failed: ", ccc_last_error());
+}
+
 // Opt-in per-kernel timing with HIP events (device of the current stream).
 void set_profiling(bool on) {
     const int dev = c10::hip::current_device();
@@ -497,6 +505,9 @@ PYBIND11_MODULE(_cccollisions, m) {
           py::arg("triangles"), py::arg("triangles_next"), py::arg("triangles_mass"));
     m.def("set_fixed_capacity", &set_fixed_capacity,
           "Keep the reference candidate capacity (max_candidates_per_triangle * N) instead of growing it", py::arg("on"));
+    m.def("set_pair_chunk", &set_pair_chunk,
+          "Candidate pairs per narrow-phase chunk over a call's batch (0: default 2^25); results do not depend on it",
+          py::arg("pairs"));
     m.def("set_profiling", &set_profiling, "Time every kernel launch with HIP events (clears totals)", py::arg("on"));
     m.def("kernel_times", &kernel_times, "Per-kernel {name: (total_ms, launches)} since profiling was enabled");
     m.def("last_counts", &last_counts, "Per-batch [K, M, V, R] stage counts of the last query on this thread");

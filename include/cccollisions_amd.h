@@ -78,6 +78,13 @@ const char *ccc_status_string(ccc_status status);
 const char *ccc_last_error(void);
 int ccc_abi_version(void);
 
+/* Pairs of candidate triangles per narrow-phase chunk, over all batch elements of a call (no
+ * counterpart in the reference, whose narrow phase materialises every candidate's
+ * tetrahedra at once, src/collisions_continuous.cu:887-1005).  The sub-primitive items of
+ * one chunk (at most 15 per pair, 8 bytes each) are the largest workspace buffer; 0 restores
+ * the default of 2^25 pairs (4 GiB of items).  Results do not depend on it. */
+ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs);
+
 /* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,
  * every kernel launch of a call is bracketed by HIP events on the call's stream and the
  * elapsed times are accumulated per kernel name.  Enabling or disabling clears the totals. */

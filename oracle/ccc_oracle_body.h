@@ -14,11 +14,15 @@
  *                        enumeration (exact predicate) as a set.
  *   orc_ccd            - coefficients equal a sympy expansion of the determinant; roots
  *                        agree with mpmath 50-digit roots of the same coefficients; the
- *                        linear-branch drop, quadratic branch and root window are pinned
- *                        by hand cases; whole pipeline checked on analytic vertex-face and
- *                        edge-edge crossings at known times of impact.
- *   orc_impulses       - hand-derived two-body cases (head-on vertex/face, crossing edges)
- *                        with closed-form impulses; EE momentum invariant with equal masses.
+ *                        linear-branch drop, quadratic branch, root window (float32 roots
+ *                        rounded before the window) and duplicate roots are pinned by hand
+ *                        cases; whole pipeline checked on constructed crossings of every
+ *                        sub-primitive type at known times of impact.
+ *   orc_impulses       - constructed two-triangle scenes for all 15 types, three each,
+ *                        with unequal corner masses (tests/golden/ccd_typed_cases.json):
+ *                        impulses, corrections and counts from the construction (contact
+ *                        weights, normal, normal velocity); a hand-derived vertex-face
+ *                        case; EE momentum invariant with equal masses.
  *   orc_proximity      - numpy brute-force distances (float-rounded) on small meshes.
  *   orc_static_collisions - textbook SAT cases (separated, touching, interpenetrating,
  *                        shared vertex) and a numpy SAT re-implementation.
@@ -267,10 +271,13 @@ static inline void FN(cubic_coeffs)(const MTETT *t, REAL *out) {
 }
 
 /* collisions_continuous.cu:448-464 add_root: window [-EPSILON, 1+EPSILON], EPSILON = 1e-16
- * (flags.h:36-38); up to three slots; the slot stores a REAL. */
+ * (flags.h:36-38); up to three slots.  The reference's parameter is `T root`, so the double
+ * root is rounded to REAL at the call and the window is tested on the rounded value (float:
+ * a root in (1, 1 + 2^-24] rounds to 1.0f and is kept). */
 static inline void FN(add_root)(REAL *slots, double root, int *n) {
-    if (root < -1e-16 || root > 1 + 1e-16) return;
-    if (*n < 3) slots[*n] = (REAL)root;
+    const REAL r = (REAL)root;
+    if ((double)r < -1e-16 || (double)r > 1 + 1e-16) return;
+    if (*n < 3) slots[*n] = r;
     (*n)++;
 }
 

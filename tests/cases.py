@@ -41,3 +41,65 @@ def garment(body_grid=8, rings=24, segments=23, seed=3, dtype=np.float64):
     sc = scenes.garment_on_body(body_grid, rings, segments, seed=seed)
     t0, t1 = sc.soups()
     return t0.astype(dtype), t1.astype(dtype), sc.corner_masses().astype(dtype)
+
+
+# ---- constructed scenes for the solver's edge cases ----------------------------------------
+
+FACE = np.array([[0.0, 0.0, 0.0], [4.0, 0.0, 0.0], [0.0, 4.0, 0.0]])
+CENTER = np.array([1.0, 1.0, 0.0])
+FAR = np.array([[2.0, 0.5, 3.0], [0.25, 2.0, 2.5]])     # the moving triangle's other two corners
+
+
+def _face_at(t, sx, sy, dz=0.0):
+    """Face corners scaled about CENTER by (1 + sx t, 1 + sy t) in x / y and moved by dz t."""
+    f = CENTER + (FACE - CENTER) * np.array([1 + sx * t, 1 + sy * t, 1.0])
+    f[:, 2] += dz * t
+    return f
+
+
+def unit_root_scene_f32(oracle, seed=11, trials=4000):
+    """Float32 scene (vertex 0 of triangle 0 against the face of triangle 1, type 12) whose
+    cubic, built from the float32 coefficients, has its root in [0, 1 + 2^-24) strictly
+    above 1: the reference rounds it to 1.0f before its [-1e-16, 1 + 1e-16] window test
+    (collisions_continuous.cu:448-464, add_root<float>) and keeps it.  The roots of the
+    float32 coefficients are located with mpmath at 60 digits; the search is seeded.
+    Returns (t0, t1, root) with t0, t1 [2, 3, 3] float32 and the exact root."""
+    import mpmath
+    mpmath.mp.dps = 60
+    rng = np.random.default_rng(seed)
+    y0 = _face_at(0.0, 0.5, 0.25).astype(np.float32)
+    y1 = _face_at(1.0, 0.5, 0.25).astype(np.float32)
+    lo, hi = mpmath.mpf(1) + mpmath.mpf(2) ** -44, mpmath.mpf(1) + mpmath.mpf(2) ** -24 - mpmath.mpf(2) ** -44
+    for _ in range(trials):
+        z0 = np.float32(rng.uniform(0.2, 0.4))
+        V = np.float32(z0 / (1.0 + 2.0 ** -25))
+        q = np.array([1.2 + 0.3 * rng.random(), 1.1 + 0.3 * rng.random(), float(z0)], np.float32)
+        vel = np.array([0.0, 0.0, -V], np.float32)
+        x0 = np.stack([q, q + FAR[0].astype(np.float32), q + FAR[1].astype(np.float32)]).astype(np.float32)
+        x1 = (x0 + vel).astype(np.float32)
+        t0 = np.stack([x0, y0])
+        t1 = np.stack([x1, y1])
+        d = (t1 - t0).astype(np.float32)
+        tet = np.concatenate([[t0[0, 0], t0[1, 0], t0[1, 1], t0[1, 2]], [d[0, 0], d[1, 0], d[1, 1], d[1, 2]]])
+        coef = oracle.cubic_coeffs(tet.astype(np.float32))
+        roots = mpmath.polyroots([mpmath.mpf(float(c)) for c in coef], maxsteps=400, extraprec=400)
+        for r in roots:
+            if abs(mpmath.im(r)) < mpmath.mpf(10) ** -40 and lo < mpmath.re(r) < hi:
+                return t0, t1, float(mpmath.re(r))
+    raise RuntimeError("no scene found")
+
+
+def double_root_scene(dtype=np.float64):
+    """Vertex 0 of triangle 0 above the face of triangle 1; the face shrinks to the point
+    CENTER at t = 1/4 (xy scaled by 1 - 4t) while both move down by the same amount, so the
+    vertex-face polynomial is exactly quadratic (every delta difference lies in the xy plane)
+    with the double root 1/4: c3 = 0, D = c1^2 - 4 c2 c0 = 0.  At t = 1/4 the face is a point,
+    the barycentric test divides 0 by 0 and the NaN passes (collisions_continuous.cu:615-637),
+    so the reference reports the collision once per root slot: twice (:743-763)."""
+    z0, Z = 0.25, 0.5
+    y0 = _face_at(0.0, -4.0, -4.0)
+    y1 = _face_at(1.0, -4.0, -4.0, dz=-Z)
+    q = np.array([1.25, 1.125, z0])
+    x0 = np.stack([q, q + FAR[0], q + FAR[1]])
+    x1 = x0 + np.array([0.0, 0.0, -Z])
+    return np.stack([x0, y0]).astype(dtype), np.stack([x1, y1]).astype(dtype)

@@ -83,12 +83,15 @@ def test_solver_branches(oracle):
 
 
 def test_flatten_window_and_duplicates(oracle):
-    """flatten keeps 0 <= r <= 1 only (collisions_continuous.cu:743-763); a root reported
-    in two slots is emitted twice."""
-    coef = np.poly([0.25, 0.25, 3.0])            # double root at 0.25
-    sl = oracle.solve_cubic(coef)
-    inside = [s for s in sl if 0 <= s <= 1]
-    assert len(inside) in (0, 2)
+    """A double root fills two slots (collisions_continuous.cu:537-547: quadratic branch with
+    D = 0 exactly, x1 = x2); the flatten keeps 0 <= r <= 1 only (:743-763), so it emits it
+    twice (tests/test_oracle_typed.py checks the two rows end to end)."""
+    # (t - 1/4)^2: c3 = 0, D = 0.25 - 4 * 0.0625 = 0 exactly
+    assert oracle.solve_cubic([0.0, 1.0, -0.5, 0.0625]).tolist() == [0.25, 0.25, -1.0]
+    # roots at -1/2 and 3/2 are stored only inside [-1e-16, 1 + 1e-16]; 1/2 is kept
+    assert oracle.solve_cubic([0.0, 1.0, -0.0, -0.25]).tolist() == [0.5, -1.0, -1.0]
+    sl = oracle.solve_cubic([0.0, 4.0, -4.0, -3.0])     # 4 (t - 3/2)(t + 1/2)
+    assert (sl == -1).all()
 
 
 def test_dd_cube_is_rounded_cube(oracle):
