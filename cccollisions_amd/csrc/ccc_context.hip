@@ -12,6 +12,7 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
 Workspace::~Workspace() {
+    if (arena_) return;
     for (void *p : ptrs_) {
         if (ctx_ && ctx_->free_fn) ctx_->free_fn(ctx_->user, p, (void *)s_);
         else (void)hipFreeAsync(p, s_);
@@ -19,6 +20,13 @@ Workspace::~Workspace() {
 }
 
 void Workspace::release(void *p) {
+    if (arena_) {
+        if (!marks_.empty() && arena_ + marks_.back() == static_cast<char *>(p)) {
+            used_ = marks_.back();
+            marks_.pop_back();
+        }
+        return;
+    }
     for (size_t i = 0; i < ptrs_.size(); ++i) {
         if (ptrs_[i] != p) continue;
         if (ctx_ && ctx_->free_fn) ctx_->free_fn(ctx_->user, p, (void *)s_);
@@ -31,6 +39,19 @@ void Workspace::release(void *p) {
 void *Workspace::get(size_t bytes) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~(size_t)255;
+    if (arena_) {
+        if (bytes > arena_left()) {
+            failed_ = true;
+            set_error("the call needs more than the " + std::to_string(arena_bytes_) +
+                      "-byte scratch buffer: " + std::to_string(used_) + " bytes in use, " + std::to_string(bytes) +
+                      " more requested");
+            return nullptr;
+        }
+        marks_.push_back(used_);
+        void *p = arena_ + used_;
+        used_ += bytes;
+        return p;
+    }
     void *p = nullptr;
     if (ctx_ && ctx_->alloc) {
         p = ctx_->alloc(ctx_->user, bytes, (void *)s_);
@@ -95,6 +116,11 @@ ccc_status ccc_context_create(ccc_context **out, int device, ccc_alloc_fn alloc,
     c->free_fn = free_fn;
     c->user = user;
     c->num_cus = cus > 0 ? cus : 256;
+    ccc_status st = ccc::scratch_address(device, &c->scratch);
+    if (st != CCC_OK) {
+        delete c;
+        return st;
+    }
     *out = c;
     return CCC_OK;
 }

@@ -20,6 +20,8 @@ struct ccc_context {
     int num_cus;
     // pairs per narrow-phase chunk over the whole batch (0: the default, 2^25)
     int64_t pair_chunk = 0;
+    // this device's copy of the allocation-free entry point's scratch array
+    void *scratch = nullptr;
     // opt-in kernel timing (HIP events on the call's stream), accumulated per kernel name
     bool profiling = false;
     std::mutex prof_mu;
@@ -34,21 +36,32 @@ void clear_error();
 // Per-call device workspace.  Buffers come from the context's allocator (or
 // hipMallocAsync on the call's stream) and are released when the Workspace goes out of
 // scope; the call has synchronised its stream before that happens.
+// Arena mode (arena != NULL): buffers are carved from [arena, arena + arena_bytes) in
+// order, 256-byte aligned, and nothing is allocated or freed; a request that does not fit
+// fails like an allocation failure.
 class Workspace {
   public:
-    Workspace(ccc_context *ctx, hipStream_t s) : ctx_(ctx), s_(s) {}
+    Workspace(ccc_context *ctx, hipStream_t s, void *arena = nullptr, size_t arena_bytes = 0)
+        : ctx_(ctx), s_(s), arena_(static_cast<char *>(arena)), arena_bytes_(arena_bytes) {}
     ~Workspace();
     void *get(size_t bytes);
     template <typename P> P *get_as(size_t count) { return static_cast<P *>(get(count * sizeof(P))); }
-    // Returns a buffer obtained from get() before the Workspace ends (stream-ordered).
+    // Returns a buffer obtained from get() before the Workspace ends (stream-ordered).  In
+    // arena mode only the most recent buffer can be returned (its bytes become free again).
     void release(void *p);
     bool failed() const { return failed_; }
+    bool arena() const { return arena_ != nullptr; }
+    // Arena mode: bytes not yet handed out (after 256-byte alignment of the next request).
+    size_t arena_left() const { return arena_bytes_ > used_ ? arena_bytes_ - used_ : 0; }
 
   private:
     ccc_context *ctx_;
     hipStream_t s_;
     std::vector<void *> ptrs_;
     bool failed_ = false;
+    char *arena_ = nullptr;
+    size_t arena_bytes_ = 0, used_ = 0;
+    std::vector<size_t> marks_;   // arena mode: offset of each buffer handed out
 };
 
 // Records HIP events around kernel launches when the context profiles; collect() runs
@@ -144,6 +157,10 @@ struct alignas(64) DevCounters {
     unsigned int pad[5];
 };
 static_assert(sizeof(DevCounters) == 64, "DevCounters is one 64-byte line");
+
+// Device address of the allocation-free entry point's scratch array on `device` (looked up
+// once per device with that device current, then cached).
+ccc_status scratch_address(int device, void **out);
 
 // Fill kernels (used instead of hipMemsetAsync so that the calls stay capturable).
 ccc_status fill_u64(void *p, int64_t count, uint64_t value, hipStream_t s);

@@ -1410,6 +1410,8 @@ struct Query {
     ccc_counts *counts;
     uint32_t flags;
     hipStream_t s;
+    void *arena;            // non-NULL: every workspace buffer is carved from this block
+    size_t arena_bytes;
 };
 
 // End of a pair chunk: its sub-primitive items are done; the next chunk refills the list.
@@ -1445,8 +1447,8 @@ static ccc_status run_query(const Query &q) {
     hipStream_t s = q.s;
     const int64_t B = q.B, n = q.n, rows = 2 * n - 1;
     const int64_t nint = n > 1 ? n - 1 : 1;
-    Workspace ws(q.ctx, s);
-    KernelTimer tm(q.ctx, s);
+    Workspace ws(q.ctx, s, q.arena, q.arena_bytes);
+    KernelTimer tm(q.arena ? nullptr : q.ctx, s);   // no event objects in arena mode
     DevCounters *ctr = ws.get_as<DevCounters>(B);
     int *claims = ws.get_as<int>(B * n);
     int *climb = ws.get_as<int>(B * nint);
@@ -1603,11 +1605,21 @@ static ccc_status run_query(const Query &q) {
         // chunk bounds the pairs of all batch elements together (the item buffer holds B
         // chunks), so the workspace does not grow with the batch
         const int64_t budget = q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
-        const int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, budget / B), 1);
-        const int64_t capM = 15 * chunk;
-        int64_t *items = ws.get_as<int64_t>(B * capM);
+        int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, budget / B), 1);
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
+        if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        if (ws.arena()) {
+            // the item list takes what the arena has left (results do not depend on the chunk)
+            const int64_t fit = (int64_t)(ws.arena_left() / ((size_t)B * 15 * sizeof(int64_t)));
+            if (fit < 1) {
+                set_error("the scratch buffer has no room left for the sub-primitive items of one candidate pair");
+                return CCC_ERR_NO_MEMORY;
+            }
+            chunk = std::min(chunk, fit);
+        }
+        const int64_t capM = 15 * chunk;
+        int64_t *items = ws.get_as<int64_t>(B * capM);
         const unsigned gS = persistent_grid(q.ctx, (chunk + kSubprimPairs - 1) / kSubprimPairs, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
         const unsigned gCcdC = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_COLLECT>());
@@ -1785,6 +1797,34 @@ static ccc_status packed_impulses(ccc_context *ctx, const void *t0, const void *
     return run_query<float>(q);
 }
 
+// Scratch of the allocation-free entry point (ccc_compute_impulses_noalloc): every buffer
+// of a call is carved from this array.  Each GPU that loads this code object holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the array's name on the host is a
+// stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[CCC_NOALLOC_SCRATCH_BYTES];
+
+ccc_status scratch_address(int device, void **out) {
+    static std::mutex mu;
+    static std::map<int, void *> cached;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cached.find(device);
+    if (it != cached.end()) {
+        *out = it->second;
+        return CCC_OK;
+    }
+    int prev = 0;
+    CCC_HIP_TRY(hipGetDevice(&prev));
+    CCC_HIP_TRY(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    cached[device] = p;
+    *out = p;
+    return CCC_OK;
+}
+
 }  // namespace ccc
 
 using ccc::Narrow;
@@ -1958,6 +1998,31 @@ ccc_status ccc_compute_impulses(ccc_context *ctx, const void *bboxes, const int6
     (void)max_collisions;
     return impulses(ctx, bboxes, tree, triangles, triangles_next, masses, nullptr, batch, n, dtype, max_candidates, dv, dx, cnt,
                     counts, flags, stream);
+}
+
+ccc_status ccc_compute_impulses_noalloc(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,
+                                        const void *triangles_next, const void *masses, int64_t batch, int64_t n, int32_t dtype,
+                                        int32_t max_candidates, int32_t max_collisions, void *dv, void *dx, int32_t *cnt,
+                                        ccc_counts *counts, uint32_t flags, void *stream) {
+    (void)max_collisions;
+    ccc::clear_error();
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    if (!ctx->scratch) { ccc::set_error("the context has no scratch address"); return CCC_ERR_HIP; }
+    Query q = base_query(ctx, bboxes, tree, triangles, batch, n, dtype, flags, stream);
+    q.t1 = triangles_next;
+    q.masses = masses;
+    q.thr = 0.0;
+    q.cap_candidates = (int64_t)max_candidates * n;
+    q.out_cap = 0;
+    q.narrow = Narrow::CCD_IMPULSE;
+    q.dv = dv;
+    q.dx = dx;
+    q.cnt = cnt;
+    q.counts = counts;
+    q.arena = ctx->scratch;
+    q.arena_bytes = CCC_NOALLOC_SCRATCH_BYTES;
+    if (max_candidates < 1) return ccc::bad_arg("max_candidates_per_triangle must be >= 1");
+    return ccc::run(q);
 }
 
 ccc_status ccc_compute_impulses_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree, const void *triangles,

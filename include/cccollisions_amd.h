@@ -208,6 +208,27 @@ ccc_status ccc_compute_impulses_packed(ccc_context *ctx, const void *triangles, 
                                        int32_t dtype, int32_t max_candidates, void *dv, void *dx, int32_t *cnt,
                                        ccc_counts *counts, uint32_t flags, void *stream);
 
+/* Size of the device scratch array behind ccc_compute_impulses_noalloc (256 MiB per GPU). */
+#define CCC_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* ccc_compute_impulses without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results and statuses as
+ * ccc_compute_impulses; the caller passes every input and output, and the call's workspace
+ * (counters, traversal nodes, candidate pairs, sub-primitive items) is carved from one
+ * static device array of CCC_NOALLOC_SCRATCH_BYTES per GPU, held from the first context
+ * created on that GPU until the process ends.  A call whose workspace does not fit returns
+ * CCC_ERR_NO_MEMORY with a message (the candidate pairs alone take max_candidates * n * 8
+ * bytes per batch element); the sub-primitive item list shrinks to what is left, which does
+ * not change the results.  Every call on one GPU uses that GPU's single array: calls that
+ * overlap on one GPU (several streams or host threads) race and must be serialised by the
+ * caller.  The call still synchronises its stream, as ccc_compute_impulses does. */
+ccc_status ccc_compute_impulses_noalloc(ccc_context *ctx, const void *bboxes, const int64_t *tree,
+                                        const void *triangles, const void *triangles_next,
+                                        const void *masses, int64_t batch, int64_t n, int32_t dtype,
+                                        int32_t max_candidates, int32_t max_collisions, void *dv, void *dx,
+                                        int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream);
+
 /* Replaces collision_impulses_from_bbox_tree_partial -- src/collisions_impulses.cu:1402-1734
  * (binding src/cccollisions.cpp:336-370). */
 ccc_status ccc_compute_impulses_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree,
