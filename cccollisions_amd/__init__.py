@@ -36,6 +36,7 @@ compute_impulses_from_collisions = _ext.compute_impulses_from_collisions
 compute_impulses_packed = _ext.compute_impulses_packed
 set_fixed_capacity = _ext.set_fixed_capacity
 set_pair_chunk = _ext.set_pair_chunk
+set_morton_mode = _ext.set_morton_mode
 last_counts = _ext.last_counts
 set_profiling = _ext.set_profiling
 kernel_times = _ext.kernel_times
@@ -51,5 +52,5 @@ ENTRY_POINTS = (
 # beyond the reference's API: many meshes of different sizes in one pass of launches
 EXTENSIONS = ("compute_impulses_packed",)
 
-__all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "set_pair_chunk", "last_counts", "set_profiling",
+__all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "set_pair_chunk", "set_morton_mode", "last_counts", "set_profiling",
                                                     "kernel_times"]

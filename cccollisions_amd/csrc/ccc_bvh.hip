@@ -1,5 +1,8 @@
 // BVH construction for gfx950: leaf (swept) boxes + Morton codes, a stable LDS radix
-// sort of (code, triangle id), the Karras radix tree and an atomic bottom-up refit.
+// sort of (code, triangle id), the Karras radix tree and a bottom-up refit.
+// Morton modes (per context): the reference's (infinite scene box: every code is 0, so the
+// sort is the identity and is skipped), or a scene box fitted to the leaf boxes with 30-bit
+// (10 bits per axis) or 63-bit (21 bits per axis) codes.
 // Output is the reference wire format (src/build_bvh.cu:309-378):
 //   bboxes [B, 2n-1, 6] (min xyz, max xyz), tree [B, 2n-1, 4] int64 (parent, left, right,
 //   id), internal rows 0..n-2 (row 0 = root), leaf rows n-1..2n-2 in sorted order.
@@ -43,14 +46,40 @@ __device__ __forceinline__ uint32_t morton30(float x, float y, float z) {
     return spread_bits10((uint32_t)x) * 4u + spread_bits10((uint32_t)y) * 2u + spread_bits10((uint32_t)z);
 }
 
-// Leaf boxes (build_bvh.cu:212-219, :452-460) and Morton codes of the centroids
-// (build_bvh.cu:26-71).  The scene box is the union of all leaf boxes with the reduction's
-// initial value AABB<T>() = [-MAX, +MAX] (build_bvh.cu:243-244, :505-506; aabb.cu:25-33);
-// min/max against +-MAX leave that initial value unchanged whatever the boxes are, so the
-// scene box is [-MAX, +MAX] and is used here directly.
-template <typename T, bool MOTION>
+// 63-bit codes (not in the reference): 21 bits per axis, the same quantisation (scale by
+// 2^21, clamp to [0, 2^21 - 1] with fmin/fmax, truncate) and interleave order.
+__device__ __forceinline__ uint64_t spread_bits21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x001f00000000ffffull;
+    v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+__device__ __forceinline__ uint64_t morton63(double x, double y, double z) {
+    x = fmin(fmax(x * 2097152.0, 0.0), 2097151.0);
+    y = fmin(fmax(y * 2097152.0, 0.0), 2097151.0);
+    z = fmin(fmax(z * 2097152.0, 0.0), 2097151.0);
+    return spread_bits21((uint64_t)x) * 4u + spread_bits21((uint64_t)y) * 2u + spread_bits21((uint64_t)z);
+}
+__device__ __forceinline__ uint64_t morton63(float x, float y, float z) {
+    x = fminf(fmaxf(x * 2097152.0f, 0.0f), 2097151.0f);
+    y = fminf(fmaxf(y * 2097152.0f, 0.0f), 2097151.0f);
+    z = fminf(fmaxf(z * 2097152.0f, 0.0f), 2097151.0f);
+    return spread_bits21((uint64_t)x) * 4u + spread_bits21((uint64_t)y) * 2u + spread_bits21((uint64_t)z);
+}
+
+// Leaf boxes (build_bvh.cu:212-219, :452-460), the triangle ids in input order and, for the
+// fitted modes, the centroids the Morton codes are computed from (build_bvh.cu:26-71).
+// The reference's scene box is the union of all leaf boxes with the reduction's initial
+// value AABB<T>() = [-MAX, +MAX] (build_bvh.cu:243-244, :505-506; aabb.cu:25-33); min/max
+// against +-MAX leave that initial value unchanged whatever the boxes are, so the scene box
+// is [-MAX, +MAX], every normalised centroid is ~0.5 * 2^-1023 or smaller and every code is
+// 0: the stable sort then keeps the input order, and the compat mode writes no codes.
+template <typename T, bool MOTION, bool FIT>
 __global__ void k_leaf_boxes(const T *__restrict__ t0, const T *__restrict__ t1, int64_t total,
-                             int64_t n, T *__restrict__ boxes, uint32_t *__restrict__ codes,
+                             int64_t n, T *__restrict__ boxes, T *__restrict__ cent,
                              int32_t *__restrict__ ids) {
     int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= total) return;
@@ -67,12 +96,72 @@ __global__ void k_leaf_boxes(const T *__restrict__ t0, const T *__restrict__ t1,
     }
     T *o = boxes + 6 * gi;
     o[0] = bx.mn.x; o[1] = bx.mn.y; o[2] = bx.mn.z; o[3] = bx.mx.x; o[4] = bx.mx.y; o[5] = bx.mx.z;
-    const T smin = -Lim<T>::max(), smax = Lim<T>::max();
-    T x = (c.x - smin) / (smax - smin);
-    T y = (c.y - smin) / (smax - smin);
-    T z = (c.z - smin) / (smax - smin);
-    codes[gi] = morton30(x, y, z);
+    if (FIT) {
+        T *q = cent + 3 * gi;
+        q[0] = c.x; q[1] = c.y; q[2] = c.z;
+    }
     ids[gi] = (int32_t)(gi % n);
+}
+
+// Fitted scene box of each batch element: the union of its leaf boxes (min / max are exact
+// and order-independent, NaN coordinates are ignored as by the leaf boxes' fmin / fmax).
+// Stage 1: each block of element b reduces a strided share of its leaves to one box.
+constexpr int kSceneBlocks = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void k_scene_partial(const T *__restrict__ boxes_all, int64_t n, T *__restrict__ part_all) {
+    __shared__ T red[6][256];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const T *boxes = boxes_all + (int64_t)b * n * 6;
+    // the empty box (+inf, -inf) is the identity of the union
+    const T inf = (T)INFINITY;
+    T acc[6] = {inf, inf, inf, -inf, -inf, -inf};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + tid; i < n; i += (int64_t)gridDim.x * 256) {
+        const T *p = boxes + 6 * i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { acc[k] = tmin(acc[k], p[k]); acc[k + 3] = tmax(acc[k + 3], p[k + 3]); }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][tid] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                red[k][tid] = tmin(red[k][tid], red[k][tid + w]);
+                red[k + 3][tid] = tmax(red[k + 3][tid], red[k + 3][tid + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 6) part_all[((int64_t)b * gridDim.x + blockIdx.x) * 6 + tid] = red[tid][0];
+}
+
+// Stage 2 and the codes: one block per batch element unions the partial boxes, then every
+// leaf's centroid is normalised by the scene box and quantised (bvh.cu:54-62; NaN -> 0).
+template <typename T, typename K>
+__global__ __launch_bounds__(256) void k_codes(const T *__restrict__ part_all, int nparts, const T *__restrict__ cent_all,
+                                               int64_t n, K *__restrict__ codes_all) {
+    __shared__ T scene[6];
+    const int b = blockIdx.y;
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        const T *p = part_all + (int64_t)b * nparts * 6;
+        T v = p[k];
+        for (int j = 1; j < nparts; ++j) v = k < 3 ? tmin(v, p[6 * j + k]) : tmax(v, p[6 * j + k]);
+        scene[k] = v;
+    }
+    __syncthreads();
+    const V3<T> smin = mk(scene[0], scene[1], scene[2]), smax = mk(scene[3], scene[4], scene[5]);
+    const T *cent = cent_all + (int64_t)b * n * 3;
+    K *codes = codes_all + (int64_t)b * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T *c = cent + 3 * i;
+        const T x = (c[0] - smin.x) / (smax.x - smin.x);
+        const T y = (c[1] - smin.y) / (smax.y - smin.y);
+        const T z = (c[2] - smin.z) / (smax.z - smin.z);
+        if (sizeof(K) == 8) codes[i] = (K)morton63(x, y, z);
+        else codes[i] = (K)morton30(x, y, z);
+    }
 }
 
 // ---- stable LSD radix sort of (code, id), 8 bits per pass --------------------------------
@@ -80,16 +169,17 @@ __global__ void k_leaf_boxes(const T *__restrict__ t0, const T *__restrict__ t1,
 // scatter.  The scatter ranks keys inside a wave with a ballot multisplit (8 ballots give
 // the lanes holding the same digit) and across the block's four waves with LDS counters.
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K *__restrict__ keys, int64_t n, int shift,
                                                             uint32_t *__restrict__ hist, int nb) {
     __shared__ uint32_t h[256];
     const int b = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x;
     h[tid] = 0;
     __syncthreads();
-    const uint32_t *k = keys + (int64_t)b * n;
+    const K *k = keys + (int64_t)b * n;
     for (int r = 0; r < kSortRounds; ++r) {
         int64_t i = (int64_t)blk * kSortTile + r * kSortThreads + tid;
-        if (i < n) atomicAdd(&h[(k[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(uint32_t)(k[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
     hist[((int64_t)b * 256 + tid) * nb + blk] = h[tid];
@@ -121,8 +211,9 @@ __global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *__restrict__ hist,
     }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-                                                               uint32_t *__restrict__ kout, int32_t *__restrict__ vout,
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K *__restrict__ kin, const int32_t *__restrict__ vin,
+                                                               K *__restrict__ kout, int32_t *__restrict__ vout,
                                                                int64_t n, int shift, const uint32_t *__restrict__ hist, int nb) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t gbase[256];
@@ -137,9 +228,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *_
         __syncthreads();
         const int64_t i = (int64_t)blk * kSortTile + r * kSortThreads + tid;
         const bool valid = i < n;
-        const uint32_t key = valid ? kin[off + i] : 0u;
+        const K key = valid ? kin[off + i] : (K)0;
         const int32_t val = valid ? vin[off + i] : 0;
-        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
@@ -171,23 +262,32 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *_
 
 // ---- Karras radix tree ----------------------------------------------------------------------
 // bvh.cu:70-89 LongestCommonPrefix: -1 outside [0, n-1]; equal codes extend the prefix with
-// the ids' common prefix (32 + clz(id_i ^ id_j)).
-__device__ __forceinline__ int lcp_delta(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i, int64_t j) {
+// the ids' common prefix (32 + clz(id_i ^ id_j) for 32-bit keys, 64 + ... for 64-bit ones).
+// codes == NULL: every code is 0 (the reference's scene box), only the ids decide.
+template <typename K> __device__ __forceinline__ int clz_key(K x);
+template <> __device__ __forceinline__ int clz_key<uint32_t>(uint32_t x) { return __clz((int)x); }
+template <> __device__ __forceinline__ int clz_key<uint64_t>(uint64_t x) { return __clzll((long long)x); }
+
+template <typename K>
+__device__ __forceinline__ int lcp_delta(const K *codes, const int32_t *ids, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j > n - 1) return -1;
-    const uint32_t a = codes[i], c = codes[j];
-    if (a == c) return 32 + __clz((int)((uint32_t)ids[i] ^ (uint32_t)ids[j]));
-    return __clz((int)(a ^ c));
+    if (codes) {
+        const K a = codes[i], c = codes[j];
+        if (a != c) return clz_key<K>(a ^ c);
+    }
+    return 8 * (int)sizeof(K) + __clz((int)((uint32_t)ids[i] ^ (uint32_t)ids[j]));
 }
 
 // Karras 2012, Algorithm 1 (what build_bvh.cu:74-143 computes): node i finds its range
 // direction, the far end j by exponential + binary search and the split by binary search.
 // Writes left/right/self of row i and the parent field of both children's rows.
-__global__ void k_karras(const uint32_t *__restrict__ codes_all, const int32_t *__restrict__ ids_all, int64_t n,
+template <typename K>
+__global__ void k_karras(const K *__restrict__ codes_all, const int32_t *__restrict__ ids_all, int64_t n,
                          int64_t *__restrict__ tree_all, int2 *__restrict__ range_all) {
     const int b = blockIdx.y;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
-    const uint32_t *codes = codes_all + (int64_t)b * n;
+    const K *codes = codes_all ? codes_all + (int64_t)b * n : nullptr;
     const int32_t *ids = ids_all + (int64_t)b * n;
     int64_t *tree = tree_all + (int64_t)b * (2 * n - 1) * 4;
     int2 *range = range_all + (int64_t)b * (n - 1);
@@ -315,17 +415,40 @@ __global__ void k_node_boxes(const int2 *__restrict__ range_all, int64_t n, Chun
     for (int k = 0; k < 6; ++k) o[k] = acc[k];
 }
 
+// Stable LSD radix sort of B rows of n (key, id) pairs, 8 bits per pass over the key's width;
+// returns which buffers hold the result.
+template <typename K>
+static ccc_status radix_sort(K *&ka, K *&kb, int32_t *&va, int32_t *&vb, int64_t B, int64_t n, int bits, uint32_t *hist,
+                             hipStream_t s) {
+    const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL((k_sort_hist<K>), dim3(nb, B), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_sort_scan, dim3(B), dim3(1024), 0, s, hist, nb);
+        hipLaunchKernelGGL((k_sort_scatter<K>), dim3(nb, B), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, hist, nb);
+        CCC_LAUNCH_CHECK("radix sort");
+        K *tk = ka; ka = kb; kb = tk;
+        int32_t *tv = va; va = vb; vb = tv;
+    }
+    return CCC_OK;
+}
+
 template <typename T>
-static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, T *bboxes, int64_t *tree,
+static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, int morton, T *bboxes, int64_t *tree,
                         hipStream_t s) {
     Workspace ws(ctx, s);
     KernelTimer tm(ctx, s);
     const int64_t total = B * n;
     const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    const bool fit = morton != CCC_MORTON_COMPAT;
+    const size_t ksz = morton == CCC_MORTON_63 ? 8 : 4;
     T *boxes = ws.get_as<T>(total * 6);
-    uint32_t *ka = ws.get_as<uint32_t>(total), *kb = ws.get_as<uint32_t>(total);
-    int32_t *va = ws.get_as<int32_t>(total), *vb = ws.get_as<int32_t>(total);
-    uint32_t *hist = ws.get_as<uint32_t>((size_t)B * 256 * nb);
+    int32_t *va = ws.get_as<int32_t>(total);
+    T *cent = fit ? ws.get_as<T>(total * 3) : nullptr;
+    T *part = fit ? ws.get_as<T>((size_t)B * kSceneBlocks * 6) : nullptr;
+    void *ka = fit ? ws.get(total * ksz) : nullptr;
+    void *kb = fit ? ws.get(total * ksz) : nullptr;
+    int32_t *vb = fit ? ws.get_as<int32_t>(total) : nullptr;
+    uint32_t *hist = fit ? ws.get_as<uint32_t>((size_t)B * 256 * nb) : nullptr;
     int2 *range = ws.get_as<int2>((size_t)B * (n > 1 ? n - 1 : 1));
     ChunkLevels lv{};
     lv.box[0] = bboxes + (size_t)6 * (n - 1);
@@ -340,25 +463,41 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
 
     tm.begin("bvh_leaf_boxes");
-    if (t1) hipLaunchKernelGGL((k_leaf_boxes<T, true>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
-    else hipLaunchKernelGGL((k_leaf_boxes<T, false>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, ka, va);
+#define CCC_LEAF(M, F) hipLaunchKernelGGL((k_leaf_boxes<T, M, F>), dim3(grid_for(total, 256)), dim3(256), 0, s, t0, t1, total, n, boxes, cent, va)
+    if (t1) { if (fit) CCC_LEAF(true, true); else CCC_LEAF(true, false); }
+    else { if (fit) CCC_LEAF(false, true); else CCC_LEAF(false, false); }
+#undef CCC_LEAF
     tm.end();
     CCC_LAUNCH_CHECK("k_leaf_boxes");
 
-    tm.begin("bvh_sort");
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nb, B), dim3(kSortThreads), 0, s, ka, n, shift, hist, nb);
-        hipLaunchKernelGGL(k_sort_scan, dim3(B), dim3(1024), 0, s, hist, nb);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nb, B), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, hist, nb);
-        CCC_LAUNCH_CHECK("radix sort");
-        uint32_t *tk = ka; ka = kb; kb = tk;
-        int32_t *tv = va; va = vb; vb = tv;
+    // compat: every code is 0 and the stable sort keeps the input order (no codes, no sort)
+    if (fit) {
+        tm.begin("bvh_sort");
+        const unsigned gparts = (unsigned)std::min<int64_t>(kSceneBlocks, (n + 255) / 256);
+        hipLaunchKernelGGL((k_scene_partial<T>), dim3(gparts, B), dim3(256), 0, s, boxes, n, part);
+        const unsigned gc = (unsigned)std::min<int64_t>(1024, (n + 255) / 256);
+        if (ksz == 8) {
+            hipLaunchKernelGGL((k_codes<T, uint64_t>), dim3(gc, B), dim3(256), 0, s, part, (int)gparts, cent, n, (uint64_t *)ka);
+            CCC_LAUNCH_CHECK("morton codes");
+            uint64_t *a = (uint64_t *)ka, *b = (uint64_t *)kb;
+            ccc_status st = radix_sort<uint64_t>(a, b, va, vb, B, n, 64, hist, s);
+            if (st != CCC_OK) return st;
+            ka = a;
+        } else {
+            hipLaunchKernelGGL((k_codes<T, uint32_t>), dim3(gc, B), dim3(256), 0, s, part, (int)gparts, cent, n, (uint32_t *)ka);
+            CCC_LAUNCH_CHECK("morton codes");
+            uint32_t *a = (uint32_t *)ka, *b = (uint32_t *)kb;
+            ccc_status st = radix_sort<uint32_t>(a, b, va, vb, B, n, 32, hist, s);
+            if (st != CCC_OK) return st;
+            ka = a;
+        }
+        tm.end();
     }
-    tm.end();
     if (n > 1) {
-        CCC_TIMED(tm, "bvh_karras", hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, ka, va, n, tree,
-                                                       range));
+        tm.begin("bvh_karras");
+        if (ksz == 8) hipLaunchKernelGGL((k_karras<uint64_t>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, (const uint64_t *)ka, va, n, tree, range);
+        else hipLaunchKernelGGL((k_karras<uint32_t>), dim3(grid_for(n - 1, 256), B), dim3(256), 0, s, (const uint32_t *)ka, va, n, tree, range);
+        tm.end();
         CCC_LAUNCH_CHECK("k_karras");
     }
     tm.begin("bvh_refit");
@@ -378,10 +517,13 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
     return CCC_OK;
 }
 
-ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
+ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype, int morton,
                             void *bboxes, int64_t *tree, hipStream_t s) {
-    if (dtype == CCC_FLOAT64) return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, (double *)bboxes, tree, s);
-    if (dtype == CCC_FLOAT32) return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, (float *)bboxes, tree, s);
+    if (morton != CCC_MORTON_COMPAT && morton != CCC_MORTON_30 && morton != CCC_MORTON_63) return bad_arg("unknown Morton mode");
+    if (dtype == CCC_FLOAT64)
+        return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, morton, (double *)bboxes, tree, s);
+    if (dtype == CCC_FLOAT32)
+        return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, morton, (float *)bboxes, tree, s);
     return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
 }
 
@@ -394,7 +536,7 @@ static ccc_status build_entry(ccc_context *ctx, const void *t0, const void *t1, 
     if (B == 0) return CCC_OK;
     if (!t0 || !bboxes || !tree) return bad_arg("NULL tensor pointer");
     CCC_HIP_TRY(hipSetDevice(ctx->device));
-    return build_bvh_device(ctx, t0, t1, B, n, dtype, bboxes, tree, (hipStream_t)stream);
+    return build_bvh_device(ctx, t0, t1, B, n, dtype, ctx->morton, bboxes, tree, (hipStream_t)stream);
 }
 
 }  // namespace ccc

@@ -150,6 +150,14 @@ ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs) {
     return CCC_OK;
 }
 
+ccc_status ccc_context_set_morton(ccc_context *ctx, int mode) {
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    if (mode != CCC_MORTON_COMPAT && mode != CCC_MORTON_30 && mode != CCC_MORTON_63)
+        return ccc::bad_arg("Morton mode must be CCC_MORTON_COMPAT, CCC_MORTON_30 or CCC_MORTON_63");
+    ctx->morton = mode;
+    return CCC_OK;
+}
+
 ccc_status ccc_context_set_profiling(ccc_context *ctx, int on) {
     if (!ctx) return ccc::bad_arg("context is NULL");
     std::lock_guard<std::mutex> lock(ctx->prof_mu);

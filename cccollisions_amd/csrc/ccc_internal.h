@@ -22,6 +22,8 @@ struct ccc_context {
     int64_t pair_chunk = 0;
     // this device's copy of the allocation-free entry point's scratch array
     void *scratch = nullptr;
+    // Morton mode of the trees bvh / bvh_motion build (CCC_MORTON_*)
+    int morton = CCC_MORTON_COMPAT;
     // opt-in kernel timing (HIP events on the call's stream), accumulated per kernel name
     bool profiling = false;
     std::mutex prof_mu;
@@ -166,9 +168,10 @@ ccc_status scratch_address(int device, void **out);
 ccc_status fill_u64(void *p, int64_t count, uint64_t value, hipStream_t s);
 ccc_status fill_u32(void *p, int64_t count, uint32_t value, hipStream_t s);
 
-// BVH over B soups of n triangles each (t1 = NULL: static boxes) into the wire format;
-// arguments already validated, the device already current.
-ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,
+// BVH over B soups of n triangles each (t1 = NULL: static boxes) into the wire format, with
+// Morton mode `morton` (CCC_MORTON_*); arguments already validated, the device already
+// current.
+ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype, int morton,
                             void *bboxes, int64_t *tree, hipStream_t s);
 
 }  // namespace ccc

@@ -1772,7 +1772,8 @@ static ccc_status packed_impulses(ccc_context *ctx, const void *t0, const void *
     CCC_HIP_TRY(hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * (meshes + 1), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_mesh_end, dim3(grid_for(n, 256)), dim3(256), 0, s, d_off, meshes, n, mesh_end);
     CCC_LAUNCH_CHECK("k_mesh_end");
-    ccc_status st = build_bvh_device(ctx, t0, t1, 1, n, dtype, bboxes, tree, s);
+    // the packed traversal needs leaf position = triangle index (the reference's codes)
+    ccc_status st = build_bvh_device(ctx, t0, t1, 1, n, dtype, CCC_MORTON_COMPAT, bboxes, tree, s);
     if ((st & CCC_ERR_MASK) != CCC_OK) return st;
     Query q{};
     q.ctx = ctx;

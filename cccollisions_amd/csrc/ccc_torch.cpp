@@ -432,6 +432,15 @@ void set_pair_chunk(int64_t pairs) {
     TORCH_CHECK(st == CCC_OK, "set_pair_chunk failed: ", ccc_last_error());
 }
 
+// Morton mode of the trees bvh / bvh_motion build on the current device (0: the reference's,
+// 1: fitted scene box with 30-bit codes, 2: fitted with 63-bit codes).
+void set_morton_mode(int mode) {
+    const int dev = c10::hip::current_device();
+    auto t = at::empty({0}, at::TensorOptions().device(at::Device(at::kCUDA, dev)));
+    ccc_status st = ccc_context_set_morton(context_for(t), mode);
+    TORCH_CHECK(st == CCC_OK, "set_morton_mode failed: ", ccc_last_error());
+}
+
 // Opt-in per-kernel timing with HIP events (device of the current stream).
 void set_profiling(bool on) {
     const int dev = c10::hip::current_device();
@@ -508,6 +517,10 @@ PYBIND11_MODULE(_cccollisions, m) {
     m.def("set_pair_chunk", &set_pair_chunk,
           "Candidate pairs per narrow-phase chunk over a call's batch (0: default 2^25); results do not depend on it",
           py::arg("pairs"));
+    m.def("set_morton_mode", &set_morton_mode,
+          "Morton codes of the trees bvh / bvh_motion build: 0 the reference's (default), 1 fitted scene box with "
+          "30-bit codes, 2 fitted with 63-bit codes; queries give the same sets on every mode",
+          py::arg("mode"));
     m.def("set_profiling", &set_profiling, "Time every kernel launch with HIP events (clears totals)", py::arg("on"));
     m.def("kernel_times", &kernel_times, "Per-kernel {name: (total_ms, launches)} since profiling was enabled");
     m.def("last_counts", &last_counts, "Per-batch [K, M, V, R] stage counts of the last query on this thread");

@@ -85,6 +85,24 @@ int ccc_abi_version(void);
  * the default of 2^25 pairs (4 GiB of items).  Results do not depend on it. */
 ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs);
 
+/* Morton codes of the trees ccc_bvh / ccc_bvh_motion build (no counterpart in the reference,
+ * whose scene box reduction starts from [-MAX, +MAX], src/build_bvh.cu:243-244, :505-506, so
+ * that every code is 0 and the tree is the radix tree over triangle indices).
+ *   CCC_MORTON_COMPAT (default): the reference's tree, bit for bit.
+ *   CCC_MORTON_30: scene box fitted to the leaf boxes, 30-bit codes as in
+ *                  src/bvh.cu:54-62 (10 bits per axis); ties broken by triangle id.
+ *   CCC_MORTON_63: fitted scene box, 63-bit codes (21 bits per axis).
+ * The fitted trees are valid wire-format BVHs over the same leaf boxes: every query returns
+ * the same pair and collision sets on them (rows in another order), except the partial
+ * queries, whose triangles_to_check mask is read by sorted leaf position for the query
+ * triangle (src/collisions_continuous.cu:1090), which equals the triangle id only in
+ * CCC_MORTON_COMPAT.  Faces stored in no spatial order (meshes from files, shuffled soups)
+ * give the compat tree boxes that span the whole mesh; the fitted trees do not. */
+#define CCC_MORTON_COMPAT 0
+#define CCC_MORTON_30 1
+#define CCC_MORTON_63 2
+ccc_status ccc_context_set_morton(ccc_context *ctx, int mode);
+
 /* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,
  * every kernel launch of a call is bracketed by HIP events on the call's stream and the
  * elapsed times are accumulated per kernel name.  Enabling or disabling clears the totals. */

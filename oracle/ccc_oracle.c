@@ -46,12 +46,33 @@ static uint32_t orc_morton3d_f32(float x, float y, float z) {
     return xx * 4 + yy * 2 + zz;
 }
 
+/* 63-bit variant (not in the reference, whose codes have 10 bits per axis): the same
+ * quantisation with 21 bits per axis (scale 2^21, clamp to [0, 2^21 - 1], truncate, NaN -> 0)
+ * and the same interleave order, x bits at 3b + 2, y at 3b + 1, z at 3b. */
+static uint64_t orc_expand_bits21(uint64_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+uint64_t orc_morton63_f64(double x, double y, double z) {
+    x = fmin(fmax(x * 2097152.0, 0.0), 2097151.0);
+    y = fmin(fmax(y * 2097152.0, 0.0), 2097151.0);
+    z = fmin(fmax(z * 2097152.0, 0.0), 2097151.0);
+    return orc_expand_bits21((uint64_t)x) * 4 + orc_expand_bits21((uint64_t)y) * 2 + orc_expand_bits21((uint64_t)z);
+}
+static uint64_t orc_morton63_f32(float x, float y, float z) {
+    x = fminf(fmaxf(x * 2097152.0f, 0.0f), 2097151.0f);
+    y = fminf(fmaxf(y * 2097152.0f, 0.0f), 2097151.0f);
+    z = fminf(fmaxf(z * 2097152.0f, 0.0f), 2097151.0f);
+    return orc_expand_bits21((uint64_t)x) * 4 + orc_expand_bits21((uint64_t)y) * 2 + orc_expand_bits21((uint64_t)z);
+}
+
 /* Stable LSD radix sort of (code, id) by code -- the contract of thrust::sort_by_key on
- * unsigned keys (build_bvh.cu:274-275, :536-537). */
-static void orc_stable_sort_u32(uint32_t *keys, int32_t *vals, int64_t n) {
-    uint32_t *k2 = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+ * unsigned keys (build_bvh.cu:274-275, :536-537); 64-bit keys, 8 bits per pass. */
+static void orc_stable_sort_u64(uint64_t *keys, int32_t *vals, int64_t n) {
+    uint64_t *k2 = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
     int32_t *v2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-    for (int pass = 0; pass < 4; ++pass) {
+    for (int pass = 0; pass < 8; ++pass) {
         int sh = 8 * pass;
         int64_t hist[257];
         memset(hist, 0, sizeof(hist));
@@ -61,7 +82,7 @@ static void orc_stable_sort_u32(uint32_t *keys, int32_t *vals, int64_t n) {
             int64_t o = hist[(keys[i] >> sh) & 255u]++;
             k2[o] = keys[i]; v2[o] = vals[i];
         }
-        memcpy(keys, k2, sizeof(uint32_t) * (size_t)n);
+        memcpy(keys, k2, sizeof(uint64_t) * (size_t)n);
         memcpy(vals, v2, sizeof(int32_t) * (size_t)n);
     }
     free(k2); free(v2);
@@ -72,20 +93,24 @@ static void orc_stable_sort_u32(uint32_t *keys, int32_t *vals, int64_t n) {
 static const int ORC_EDGE[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 
 static int orc_clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
+static int orc_clz64(uint64_t x) { return x == 0 ? 64 : __builtin_clzll(x); }
 
 /* bvh.cu:70-89 LongestCommonPrefix: -1 outside [0, n-1]; equal keys add the common
- * prefix of the ids (32 + clz(id_i ^ id_j)). */
-static int orc_delta(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i, int64_t j) {
+ * prefix of the ids.  Keys are held in 64 bits: for the reference's 30-bit codes every
+ * valid delta is then 32 larger than the reference's clz32-based one (32 + clz(id ^ id) on
+ * ties becomes 64 + clz(id ^ id)), which changes none of the comparisons Karras makes, so
+ * the trees are the same. */
+static int orc_delta(const uint64_t *codes, const int32_t *ids, int64_t n, int64_t i, int64_t j) {
     if (i < 0 || i > n - 1 || j < 0 || j > n - 1) return -1;
-    uint32_t a = codes[i], b = codes[j];
-    if (a == b) return orc_clz32(a ^ b) + orc_clz32((uint32_t)(ids[i] ^ ids[j]));
-    return orc_clz32(a ^ b);
+    uint64_t a = codes[i], b = codes[j];
+    if (a == b) return 64 + orc_clz32((uint32_t)(ids[i] ^ ids[j]));
+    return orc_clz64(a ^ b);
 }
 
 /* Textbook Karras 2012 (Algorithm 1) with the delta above; this is what
  * BuildRadixTree (build_bvh.cu:74-143) computes.  left/right are tree row indices:
  * internal g -> g, leaf p -> n-1+p. */
-int orc_karras_node(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i,
+int orc_karras_node(const uint64_t *codes, const int32_t *ids, int64_t n, int64_t i,
                     int64_t *left, int64_t *right) {
     int d = (orc_delta(codes, ids, n, i, i + 1) - orc_delta(codes, ids, n, i, i - 1)) >= 0 ? 1 : -1;
     int dmin = orc_delta(codes, ids, n, i, i - d);
@@ -112,7 +137,7 @@ int orc_karras_node(const uint32_t *codes, const int32_t *ids, int64_t n, int64_
 /* The two search loops exactly as written in build_bvh.cu:100-123: the first step of
  * each loop is tested twice and the ceil-based split loop keeps testing t = 1 while the
  * 32-bit divider doubles (it is stopped here where the int would overflow). */
-int orc_karras_node_literal(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i,
+int orc_karras_node_literal(const uint64_t *codes, const int32_t *ids, int64_t n, int64_t i,
                             int64_t *left, int64_t *right) {
     int delta_next = orc_delta(codes, ids, n, i, i + 1);
     int delta_last = orc_delta(codes, ids, n, i, i - 1);
@@ -165,6 +190,7 @@ double orc_dd_cube_lead(double b) {
 #define RSQRT_BASE sqrt
 #define RMAXV DBL_MAX
 #define MORTON(x, y, z) orc_morton3d_f64((x), (y), (z))
+#define MORTON63(x, y, z) orc_morton63_f64((x), (y), (z))
 #include "ccc_oracle_body.h"
 #undef REAL
 #undef SFX
@@ -175,6 +201,7 @@ double orc_dd_cube_lead(double b) {
 #undef RSQRT_BASE
 #undef RMAXV
 #undef MORTON
+#undef MORTON63
 
 #define REAL float
 #define SFX _f32
@@ -185,6 +212,7 @@ double orc_dd_cube_lead(double b) {
 #define RSQRT_BASE sqrtf
 #define RMAXV FLT_MAX
 #define MORTON(x, y, z) orc_morton3d_f32((x), (y), (z))
+#define MORTON63(x, y, z) orc_morton63_f32((x), (y), (z))
 #include "ccc_oracle_body.h"
 #undef REAL
 #undef SFX
@@ -195,6 +223,7 @@ double orc_dd_cube_lead(double b) {
 #undef RSQRT_BASE
 #undef RMAXV
 #undef MORTON
+#undef MORTON63
 
 /* ---- CPU baseline: one CCD + impulse pass ------------------------------------------
  * bvh_motion (orc_bvh_f64) -> traversal -> narrow phase -> impulses, parallelised with

@@ -45,7 +45,7 @@ typedef struct {
 
 #define ORC_DECL(SFX, REAL)                                                                  \
     int orc_bvh##SFX(const REAL *tris, const REAL *tris_next, int64_t n, int morton_mode,   \
-                     REAL *bboxes, int64_t *tree, uint32_t *codes_out, int32_t *ids_out);   \
+                     REAL *bboxes, int64_t *tree, uint64_t *codes_out, int32_t *ids_out);   \
     int64_t orc_traverse##SFX(const REAL *bboxes, const int64_t *tree, int64_t n,           \
                               REAL threshold, int64_t *pairs, int64_t cap);                 \
     int64_t orc_traverse_partial##SFX(const REAL *bboxes, const int64_t *tree, int64_t n,   \
@@ -77,11 +77,12 @@ ORC_DECL(_f32, float)
 
 /* Double-only helpers exposed for pinning tests. */
 double orc_dd_cube_lead(double b);
-int orc_karras_node(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i,
+int orc_karras_node(const uint64_t *codes, const int32_t *ids, int64_t n, int64_t i,
                     int64_t *left, int64_t *right);
-int orc_karras_node_literal(const uint32_t *codes, const int32_t *ids, int64_t n, int64_t i,
+int orc_karras_node_literal(const uint64_t *codes, const int32_t *ids, int64_t n, int64_t i,
                             int64_t *left, int64_t *right);
 uint32_t orc_morton3d_f64(double x, double y, double z);
+uint64_t orc_morton63_f64(double x, double y, double z);
 void orc_libm_eval(int fn, const double *x, double *y, int64_t n);
 
 /* Whole CCD+impulse pass (bvh_motion -> traverse -> narrow phase -> impulses) used as

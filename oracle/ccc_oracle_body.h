@@ -9,7 +9,11 @@
  *                        textbook Karras definition; Morton codes are all zero (invariant);
  *                        the literal reference search loops equal the textbook search for
  *                        N = 2..3000; boxes equal numpy min/max (exact) and parent boxes equal
- *                        the union of children.
+ *                        the union of children.  Fitted-box modes (1: 30-bit, 2: 63-bit codes):
+ *                        codes equal a bit-by-bit interleave of numpy-quantised centroids,
+ *                        the leaf order equals numpy's stable argsort of those codes, and
+ *                        every internal node equals the textbook Karras node rebuilt in
+ *                        Python over (code, id) keys.
  *   orc_traverse       - equals orc_bruteforce_pairs and an independent numpy O(N^2) pair
  *                        enumeration (exact predicate) as a set.
  *   orc_ccd            - coefficients equal a sympy expansion of the determinant; roots
@@ -389,10 +393,10 @@ static inline int FN(filter_collision)(const MTETT *t, int type, REAL root) {
  * BVH (src/build_bvh.cu).  Tree rows: internal 0..n-2, leaf n-1+p for sorted position p.
  */
 int FN(orc_bvh)(const REAL *tris, const REAL *tris_next, int64_t n, int morton_mode,
-                REAL *bboxes, int64_t *tree, uint32_t *codes_out, int32_t *ids_out) {
+                REAL *bboxes, int64_t *tree, uint64_t *codes_out, int32_t *ids_out) {
     if (n < 1) return -1;
     BOXT *box = (BOXT *)malloc(sizeof(BOXT) * (size_t)n);
-    uint32_t *codes = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    uint64_t *codes = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
     int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
     if (!box || !codes || !ids) { free(box); free(codes); free(ids); return -2; }
     /* ComputeTriBoundingBoxes :212-219 (+ MergeTriBoundingBoxes :452-460 for motion) */
@@ -401,7 +405,8 @@ int FN(orc_bvh)(const REAL *tris, const REAL *tris_next, int64_t n, int morton_m
         if (tris_next) box[i] = FN(box_union)(box[i], FN(tri_box)(FN(load_tri)(tris_next, i)));
     }
     /* scene box :243-244 / :505-506: reduce with init AABB<T>() = [-MAX, +MAX]
-     * (aabb.cu:25-33).  morton_mode 1 (not the reference) starts from an empty box. */
+     * (aabb.cu:25-33).  morton_mode 1 (30-bit codes) and 2 (63-bit codes), both not the
+     * reference, start from an empty box: the scene box fits the leaf boxes. */
     BOXT scene;
     if (morton_mode == 0) {
         scene.mn = FN(mk)(-RMAXV, -RMAXV, -RMAXV); scene.mx = FN(mk)(RMAXV, RMAXV, RMAXV);
@@ -423,12 +428,12 @@ int FN(orc_bvh)(const REAL *tris, const REAL *tris_next, int64_t n, int morton_m
         REAL x = (c.x - scene.mn.x) / (scene.mx.x - scene.mn.x);
         REAL y = (c.y - scene.mn.y) / (scene.mx.y - scene.mn.y);
         REAL z = (c.z - scene.mn.z) / (scene.mx.z - scene.mn.z);
-        codes[i] = MORTON(x, y, z);
+        codes[i] = morton_mode == 2 ? MORTON63(x, y, z) : (uint64_t)MORTON(x, y, z);
     }
     /* thrust::sequence + stable sort_by_key :268-279 / :530-541 */
     for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)i;
-    orc_stable_sort_u32(codes, ids, n);
-    if (codes_out) memcpy(codes_out, codes, sizeof(uint32_t) * (size_t)n);
+    orc_stable_sort_u64(codes, ids, n);
+    if (codes_out) memcpy(codes_out, codes, sizeof(uint64_t) * (size_t)n);
     if (ids_out) memcpy(ids_out, ids, sizeof(int32_t) * (size_t)n);
 
     /* BuildRadixTree :74-143 and flatten :309-378 */

@@ -81,6 +81,8 @@ def _declare(L):
         getattr(L, f).restype = C.c_int
     L.orc_morton3d_f64.argtypes = [C.c_double, C.c_double, C.c_double]
     L.orc_morton3d_f64.restype = C.c_uint32
+    L.orc_morton63_f64.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.orc_morton63_f64.restype = C.c_uint64
     L.orc_libm_eval.argtypes = [C.c_int, _P, _P, _I64]
     L.orc_impulse_pass_f64.argtypes = [_P, _P, _P, _I64, C.c_int, _I64, _P, _P, _P, C.POINTER(Counts)]
     L.orc_impulse_pass_f64.restype = C.c_int
@@ -104,7 +106,9 @@ def _ptr(a):
 
 
 def bvh(tris, tris_next=None, morton_mode=0):
-    """Reference `bvh` / `bvh_motion` for one mesh: returns (bboxes, tree, codes, ids)."""
+    """Reference `bvh` / `bvh_motion` for one mesh: returns (bboxes, tree, codes, ids).
+    morton_mode 0 is the reference (infinite scene box, all codes 0); 1 and 2 fit the scene
+    box to the leaf boxes and use 30-bit / 63-bit codes (uint64 either way)."""
     tris = _c(tris)
     n = tris.shape[0]
     sfx, _ = _sfx(tris)
@@ -112,7 +116,7 @@ def bvh(tris, tris_next=None, morton_mode=0):
         tris_next = _c(tris_next, tris.dtype)
     bb = np.empty((2 * n - 1, 6), tris.dtype)
     tree = np.empty((2 * n - 1, 4), np.int64)
-    codes = np.empty(n, np.uint32)
+    codes = np.empty(n, np.uint64)
     ids = np.empty(n, np.int32)
     rc = getattr(lib(), "orc_bvh" + sfx)(_ptr(tris), _ptr(tris_next), n, morton_mode,
                                          _ptr(bb), _ptr(tree), _ptr(codes), _ptr(ids))
@@ -288,7 +292,7 @@ def dd_cube_lead(b):
 
 
 def karras_node(codes, ids, i, literal=False):
-    codes = _c(codes, np.uint32)
+    codes = _c(codes, np.uint64)
     ids = _c(ids, np.int32)
     l, r = C.c_int64(), C.c_int64()
     f = lib().orc_karras_node_literal if literal else lib().orc_karras_node
@@ -298,6 +302,10 @@ def karras_node(codes, ids, i, literal=False):
 
 def morton3d(x, y, z):
     return lib().orc_morton3d_f64(x, y, z)
+
+
+def morton63(x, y, z):
+    return lib().orc_morton63_f64(x, y, z)
 
 
 LIBM_FUNCS = {"cbrt": 0, "acos": 1, "sin": 2, "cos": 3, "cbrtf": 4}
