@@ -40,6 +40,7 @@ set_morton_mode = _ext.set_morton_mode
 last_counts = _ext.last_counts
 set_profiling = _ext.set_profiling
 kernel_times = _ext.kernel_times
+Sim = _ext.Sim
 
 ENTRY_POINTS = (
     "bvh", "bvh_motion", "find_collisions", "find_triangle_edge_collisions",
@@ -50,7 +51,7 @@ ENTRY_POINTS = (
 )
 
 # beyond the reference's API: many meshes of different sizes in one pass of launches
-EXTENSIONS = ("compute_impulses_packed",)
+EXTENSIONS = ("compute_impulses_packed", "Sim")
 
 __all__ = list(ENTRY_POINTS) + list(EXTENSIONS) + ["set_fixed_capacity", "set_pair_chunk", "set_morton_mode", "last_counts", "set_profiling",
                                                     "kernel_times"]

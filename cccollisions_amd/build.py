@@ -24,7 +24,7 @@ OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libccc_amd.so")
 EXT_NAME = "_cccollisions"
 
-HIP_SOURCES = ["ccc_context.hip", "ccc_bvh.hip", "ccc_query.hip"]
+HIP_SOURCES = ["ccc_context.hip", "ccc_bvh.hip", "ccc_query.hip", "ccc_sim.hip"]
 ARCH = os.environ.get("CCC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -434,9 +434,9 @@ static ccc_status radix_sort(K *&ka, K *&kb, int32_t *&va, int32_t *&vb, int64_t
 
 template <typename T>
 static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, int morton, T *bboxes, int64_t *tree,
-                        hipStream_t s) {
-    Workspace ws(ctx, s);
-    KernelTimer tm(ctx, s);
+                        hipStream_t s, void *arena, size_t arena_bytes) {
+    Workspace ws(ctx, s, arena, arena_bytes);
+    KernelTimer tm(arena ? nullptr : ctx, s);
     const int64_t total = B * n;
     const int nb = (int)((n + kSortTile - 1) / kSortTile);
     const bool fit = morton != CCC_MORTON_COMPAT;
@@ -518,13 +518,30 @@ static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, i
 }
 
 ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype, int morton,
-                            void *bboxes, int64_t *tree, hipStream_t s) {
+                            void *bboxes, int64_t *tree, hipStream_t s, void *arena, size_t arena_bytes) {
     if (morton != CCC_MORTON_COMPAT && morton != CCC_MORTON_30 && morton != CCC_MORTON_63) return bad_arg("unknown Morton mode");
     if (dtype == CCC_FLOAT64)
-        return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, morton, (double *)bboxes, tree, s);
+        return build<double>(ctx, (const double *)t0, (const double *)t1, B, n, morton, (double *)bboxes, tree, s, arena, arena_bytes);
     if (dtype == CCC_FLOAT32)
-        return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, morton, (float *)bboxes, tree, s);
+        return build<float>(ctx, (const float *)t0, (const float *)t1, B, n, morton, (float *)bboxes, tree, s, arena, arena_bytes);
     return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
+}
+
+size_t build_bvh_arena_bytes(int64_t B, int64_t n, int32_t dtype, int morton) {
+    auto r = [](size_t b) { return ((b ? b : 256) + 255) & ~(size_t)255; };
+    const size_t esz = dtype == CCC_FLOAT64 ? 8 : 4, total = (size_t)(B * n);
+    const bool fit = morton != CCC_MORTON_COMPAT;
+    const size_t ksz = morton == CCC_MORTON_63 ? 8 : 4;
+    const int nb = (int)((n + kSortTile - 1) / kSortTile);
+    size_t tot = r(esz * total * 6) + r(4 * total);
+    if (fit) tot += r(esz * total * 3) + r(esz * B * kSceneBlocks * 6) + 2 * r(ksz * total) + r(4 * total) + r(4 * (size_t)B * 256 * nb);
+    tot += r(sizeof(int2) * B * (n > 1 ? n - 1 : 1));
+    int64_t count = n;
+    for (int levels = 0; levels < kChunkLevels && count > kChunk; ++levels) {
+        count = (count + kChunk - 1) / kChunk;
+        tot += r(esz * B * count * 6);
+    }
+    return tot;
 }
 
 static ccc_status build_entry(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype,

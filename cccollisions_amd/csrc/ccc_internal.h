@@ -164,6 +164,37 @@ static_assert(sizeof(DevCounters) == 64, "DevCounters is one 64-byte line");
 // once per device with that device current, then cached).
 ccc_status scratch_address(int device, void **out);
 
+// One compute_impulses pass of one mesh (or one packed set of meshes) over caller-owned,
+// resident buffers, for the resident simulation.  Nothing waits on the host and nothing is
+// allocated: workspace comes from `arena` in a fixed order (so the buffers have the same
+// addresses on every call with the same sizes and the pass can be captured in a HIP graph),
+// the candidate list keeps at most cap_pairs pairs (more are counted in ctr->cand and must
+// be detected by the caller after synchronising), the narrow phase covers [0, cap_pairs)
+// in chunks of `chunk` pairs whose kernels stop at the counts on the device, and the
+// counters stay in `ctr` (zeroed by the pass).  det_acc != NULL: the impulse sums are
+// order-independent (fixed-point limbs, kDetWords int64 per corner) and then written to dv
+// / dx, so that a pass gives the same bits however its atomics interleave.
+struct ResidentPass {
+    const void *bboxes;
+    const int64_t *tree;
+    const void *t0, *t1, *masses;
+    int64_t n;
+    int32_t dtype;
+    const int32_t *mesh_end;     // packed meshes: per triangle, the end of its mesh; else NULL
+    int2 *pairs;
+    int64_t cap_pairs;
+    int64_t chunk;
+    int ccd_blocks_per_cu;       // grid cap of k_ccd (0: 128)
+    bool stack_deep;             // traverse with the deep stack from the start
+    long long *det_acc;
+    void *dv, *dx;
+    int32_t *cnt;
+    DevCounters *ctr;
+};
+ccc_status resident_pass(ccc_context *ctx, const ResidentPass &p, void *arena, size_t arena_bytes, hipStream_t s);
+size_t resident_pass_arena_bytes(ccc_context *ctx, const ResidentPass &p);
+constexpr int kDetWordsPerCorner = 24;
+
 // Fill kernels (used instead of hipMemsetAsync so that the calls stay capturable).
 ccc_status fill_u64(void *p, int64_t count, uint64_t value, hipStream_t s);
 ccc_status fill_u32(void *p, int64_t count, uint32_t value, hipStream_t s);
@@ -172,7 +203,9 @@ ccc_status fill_u32(void *p, int64_t count, uint32_t value, hipStream_t s);
 // Morton mode `morton` (CCC_MORTON_*); arguments already validated, the device already
 // current.
 ccc_status build_bvh_device(ccc_context *ctx, const void *t0, const void *t1, int64_t B, int64_t n, int32_t dtype, int morton,
-                            void *bboxes, int64_t *tree, hipStream_t s);
+                            void *bboxes, int64_t *tree, hipStream_t s, void *arena = nullptr, size_t arena_bytes = 0);
+// Workspace bytes of build_bvh_device (all of its buffers, 256-byte rounded).
+size_t build_bvh_arena_bytes(int64_t B, int64_t n, int32_t dtype, int morton);
 
 }  // namespace ccc
 

@@ -900,6 +900,77 @@ template <typename T> struct ImpTable {
 };
 static_assert(kImpSlots == 1 << kImpBits, "the slot count is a power of two");
 
+// ---- order-independent impulse sums (deterministic mode) ----------------------------------
+// A finite value v with |v| < 2^45 is held as the integer trunc(|v| * 2^80) with v's sign,
+// split into three 42-bit limbs; every limb of every contribution is added with a 64-bit
+// integer atomic, which is exact and associative, so the sums do not depend on the order of
+// the additions (2^21 additions per limb fit in 64 bits).  Values that cannot be held set
+// flag bits in a fourth word instead: 1 NaN, 2 +inf, 4 -inf, 8 out of range; the result
+// is then what an fp64 sum gives (NaN, or the infinity) and NaN when out of range.
+// Per triangle corner: 6 components (dv xyz, dx xyz) x 4 words.
+constexpr int kDetWords = kDetWordsPerCorner;
+constexpr double kTwo42 = 4398046511104.0, kTwo45 = 35184372088832.0, kTwo80 = 1208925819614629174706176.0;
+
+__device__ __forceinline__ void det_add1(long long *a, double v) {
+    if (!(fabs(v) < kTwo45)) {       // NaN fails this test too
+        const unsigned long long f = v != v ? 1ull : v == (double)INFINITY ? 2ull : v == -(double)INFINITY ? 4ull : 8ull;
+        atomicOr(reinterpret_cast<unsigned long long *>(a + 3), f);
+        return;
+    }
+    const double sc = fabs(v) * kTwo80;                  // exact: a power-of-two scaling, < 2^125
+    const double hi = floor(sc / (kTwo42 * kTwo42));    // < 2^41; exact
+    const double r1 = sc - hi * (kTwo42 * kTwo42);      // the bits of sc below 2^84: exact
+    const double mid = floor(r1 / kTwo42);              // < 2^42
+    const double lo = floor(r1 - mid * kTwo42);         // < 2^42, fraction truncated
+    long long l0 = (long long)lo, l1 = (long long)mid, l2 = (long long)hi;
+    if (v < 0) { l0 = -l0; l1 = -l1; l2 = -l2; }
+    unsigned long long *u = reinterpret_cast<unsigned long long *>(a);
+    if (l0) atomicAdd(u + 0, (unsigned long long)l0);
+    if (l1) atomicAdd(u + 1, (unsigned long long)l1);
+    if (l2) atomicAdd(u + 2, (unsigned long long)l2);
+}
+
+template <typename T>
+__device__ __forceinline__ void det_add_corner(long long *acc, int64_t c, const V3<T> &imp, const V3<T> &imd) {
+    long long *a = acc + kDetWords * c;
+    det_add1(a + 0, (double)imp.x);
+    det_add1(a + 4, (double)imp.y);
+    det_add1(a + 8, (double)imp.z);
+    det_add1(a + 12, (double)imd.x);
+    det_add1(a + 16, (double)imd.y);
+    det_add1(a + 20, (double)imd.z);
+}
+
+// Limbs -> dv / dx of every corner: carries normalise the two low limbs into [0, 2^42),
+// then hi * 2^84 + mid * 2^42 + lo is formed in double in that order and scaled by 2^-80.
+template <typename T>
+__global__ void k_det_finalize(const long long *__restrict__ acc, int64_t corners, T *__restrict__ dv, T *__restrict__ dx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= corners) return;
+    const long long *a = acc + kDetWords * c;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        long long l0 = a[4 * k], l1 = a[4 * k + 1], l2 = a[4 * k + 2];
+        const unsigned long long f = (unsigned long long)a[4 * k + 3];
+        double v;
+        if (f & 9ull) v = (double)NAN;
+        else if ((f & 6ull) == 6ull) v = (double)NAN;
+        else if (f & 2ull) v = (double)INFINITY;
+        else if (f & 4ull) v = -(double)INFINITY;
+        else {
+            const long long c0 = l0 >> 42;
+            l0 -= c0 * (1ll << 42);
+            l1 += c0;
+            const long long c1 = l1 >> 42;
+            l1 -= c1 * (1ll << 42);
+            l2 += c1;
+            v = ((double)l2 * (kTwo42 * kTwo42) + (double)l1 * kTwo42 + (double)l0) / kTwo80;
+        }
+        T *o = k < 3 ? dv + 3 * c + k : dx + 3 * c + (k - 3);
+        *o = (T)v;
+    }
+}
+
 // Hides a pointer's provenance from the optimiser so that loads through it are not
 // merged with earlier loads of the same addresses.
 template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
@@ -949,13 +1020,13 @@ __device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
     return st;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool DET>
 __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
-                                                  DevCounters *__restrict__ ctr_all) {
+                                                  long long *__restrict__ det_all, DevCounters *__restrict__ ctr_all) {
     constexpr int NT = ccd_threads<MODE>();     // items per tile, threads per block
     constexpr int kRootCapT = 3 * NT;
     static_assert(MODE != MODE_IMPULSE || NT == kCcdThreads, "the impulse table is walked by kCcdThreads threads");
@@ -987,6 +1058,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     T *dv = MODE == MODE_IMPULSE ? dv_all + (int64_t)b * n * 9 : nullptr;
     T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
     int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
+    long long *det = DET ? det_all + (int64_t)b * n * 3 * kDetWords : nullptr;
     unsigned long long nroots = 0, ncoll = 0;
     if (tid == 0) s_fill = 0;
     __syncthreads();
@@ -1006,7 +1078,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     auto flush = [&](int f) {
         for (int base = 0; base < f; base += kFlush) {
             const int nf = f - base < kFlush ? f - base : kFlush;
-            tab.clear(tid);
+            if (!DET) tab.clear(tid);
             if (tid < nf) {
                 int64_t x, y;
                 int type;
@@ -1038,11 +1110,16 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
                     ld_vrec(opaque_ptr(tv), c, v, dvk);
                     V3<T> imp, imd;
                     corner_impulse(s_imp[jc], type, k, dvk, imp, imd);
-                    tab.add(c, imp, imd, dv, dx, cnt);
+                    if (DET) {
+                        det_add_corner(det, c, imp, imd);
+                        atomicAdd(cnt + c, 1);
+                    } else {
+                        tab.add(c, imp, imd, dv, dx, cnt);
+                    }
                 }
             }
             __syncthreads();
-            tab.drain(tid, dv, dx, cnt);
+            if (!DET) tab.drain(tid, dv, dx, cnt);
             __syncthreads();
         }
     };
@@ -1412,6 +1489,18 @@ struct Query {
     hipStream_t s;
     void *arena;            // non-NULL: every workspace buffer is carved from this block
     size_t arena_bytes;
+    // Resident passes (the resident simulation): counters, the candidate list and the
+    // deterministic impulse limbs live in the caller's memory, and with `nosync` nothing
+    // waits on the host -- the list keeps at most cap_candidates pairs (more are counted),
+    // the narrow phase walks [0, cap_candidates) in chunks whose kernels stop at the
+    // device's counts, and the counters stay on the device.  Such a pass is capturable.
+    bool nosync;
+    DevCounters *ctr_ext;   // B counters (zeroed by the pass) or NULL
+    int2 *pairs_ext;        // B * cap_candidates pairs or NULL
+    int64_t chunk_ext;      // pairs per narrow-phase chunk (0: the context's)
+    int ccd_blocks_per_cu;  // grid cap of k_ccd in blocks per CU (0: 128)
+    bool stack_deep;        // start with the deep traversal stack
+    long long *det_acc;     // non-NULL: impulse sums in fixed-point limbs (n * 3 * 6 * 4 per element)
 };
 
 // End of a pair chunk: its sub-primitive items are done; the next chunk refills the list.
@@ -1432,10 +1521,10 @@ constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim
 // then walking several tiles.  Measured on the 130k cloth (k_ccd per step): 4 blocks per CU
 // (all resident at once) 232 ms, 16 215 ms, 32 210, 128 207, 512 214 -- many short blocks
 // let the dispatcher balance the uneven tiles; past that the per-block start-up dominates.
-static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B, int block = kThreads) {
+static unsigned persistent_grid(const ccc_context *ctx, int64_t items, int64_t B, int block = kThreads, int per_cu = 128) {
     int64_t want = (items + block - 1) / block;
     if (want < 1) want = 1;                        // no items: a grid of idle blocks, never 0
-    int64_t limit = (int64_t)ctx->num_cus * 128 / (B > 0 ? B : 1);
+    int64_t limit = (int64_t)ctx->num_cus * per_cu / (B > 0 ? B : 1);
     if (limit < kXcds) limit = kXcds;
     if (want > limit) want = limit;
     want = (want + kXcds - 1) / kXcds * kXcds;    // CCC_FOR_TILES needs a multiple of kXcds
@@ -1449,7 +1538,7 @@ static ccc_status run_query(const Query &q) {
     const int64_t nint = n > 1 ? n - 1 : 1;
     Workspace ws(q.ctx, s, q.arena, q.arena_bytes);
     KernelTimer tm(q.arena ? nullptr : q.ctx, s);   // no event objects in arena mode
-    DevCounters *ctr = ws.get_as<DevCounters>(B);
+    DevCounters *ctr = q.ctr_ext ? q.ctr_ext : ws.get_as<DevCounters>(B);
     int *claims = ws.get_as<int>(B * n);
     int *climb = ws.get_as<int>(B * nint);
     int *rm = ws.get_as<int>(B * nint);
@@ -1488,6 +1577,7 @@ static ccc_status run_query(const Query &q) {
             if (sizeof(T) == 8) { st = fill_u64(q.dv, B * n * 9, 0ull, s); if (st == CCC_OK) st = fill_u64(q.dx, B * n * 9, 0ull, s); }
             else { st = fill_u32(q.dv, B * n * 9, 0u, s); if (st == CCC_OK) st = fill_u32(q.dx, B * n * 9, 0u, s); }
             if (st == CCC_OK) st = fill_u32(q.cnt, B * n * 3, 0u, s);
+            if (st == CCC_OK && q.det_acc) st = fill_u64(q.det_acc, B * n * 3 * kDetWords, 0ull, s);
             break;
     }
     if (st != CCC_OK) return st;
@@ -1519,11 +1609,15 @@ static ccc_status run_query(const Query &q) {
     if (ws.failed()) return CCC_ERR_NO_MEMORY;
     hipLaunchKernelGGL((k_qblock_boxes<T>), dim3(nqb, B), dim3(kTravThreads), 0, s, bb, n, nqb, qbox, ctr);
     CCC_LAUNCH_CHECK("k_qblock_boxes");
-    bool deep = false;   // the short stack overflowed once: run with the deep one
+    bool deep = q.stack_deep;   // the short stack overflowed once: run with the deep one
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if (pairs) ws.release(pairs);     // a re-run: the first attempt's pairs are not kept
-        pairs = ws.get_as<int2>(B * capK);
-        if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        if (q.pairs_ext) {
+            pairs = q.pairs_ext;
+        } else {
+            if (pairs) ws.release(pairs);     // a re-run: the first attempt's pairs are not kept
+            pairs = ws.get_as<int2>(B * capK);
+            if (ws.failed()) return CCC_ERR_NO_MEMORY;
+        }
         tm.begin("traverse");
 #define CCC_TRAVERSE(P, D)                                                                                   \
     hipLaunchKernelGGL((k_traverse<T, P, D>), dim3(gT, B), dim3(kTravThreads), 0, s, bb, q.tree, qnodes, recs, n, \
@@ -1538,6 +1632,7 @@ static ccc_status run_query(const Query &q) {
 #undef CCC_TRAVERSE
         tm.end();
         CCC_LAUNCH_CHECK("k_traverse");
+        if (q.nosync) break;          // the caller checks the counters after its step
         CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
         CCC_HIP_TRY(hipStreamSynchronize(s));
         unsigned flags = 0;
@@ -1562,13 +1657,15 @@ static ccc_status run_query(const Query &q) {
         if (kmax <= (unsigned long long)capK) break;
         warn_cand = true;
         if (q.flags & CCC_FLAG_FIXED_CAPACITY) break;
-        if (attempt == 1) break;
+        if (attempt == 1 || q.pairs_ext) break;   // a caller-owned list does not grow here
         capK = (int64_t)kmax;
         for (int64_t b = 0; b < B; ++b) h[b].cand = 0;
         CCC_HIP_TRY(hipMemcpyAsync(ctr, h.data(), sizeof(DevCounters) * B, hipMemcpyHostToDevice, s));
     }
     int64_t kmax = 0;
-    for (int64_t b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, std::min<int64_t>((int64_t)h[b].cand, capK));
+    if (q.nosync) kmax = capK;        // sizes must not depend on the counts
+    else
+        for (int64_t b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, std::min<int64_t>((int64_t)h[b].cand, capK));
 
     const T thr = (T)q.thr;
     const unsigned gK = persistent_grid(q.ctx, kmax, B);
@@ -1604,7 +1701,7 @@ static ccc_status run_query(const Query &q) {
         // sub-primitive items and the narrow phase run over chunks of the pair list; the
         // chunk bounds the pairs of all batch elements together (the item buffer holds B
         // chunks), so the workspace does not grow with the batch
-        const int64_t budget = q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
+        const int64_t budget = q.chunk_ext > 0 ? q.chunk_ext * B : q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
         int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, budget / B), 1);
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
@@ -1626,7 +1723,8 @@ static ccc_status run_query(const Query &q) {
         // one-wave impulse blocks, up to 128 per CU: on the 130k cloth 256 per CU took 171.6 ms
         // per step against 173.7 and 64 took 178.3, but 256 cost the 122k garment scene (fewer
         // items per pass) 23% of its step rate
-        const unsigned gCcdI = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_IMPULSE>());
+        const unsigned gCcdI = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_IMPULSE>(),
+                                               q.ccd_blocks_per_cu > 0 ? q.ccd_blocks_per_cu : 128);
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         if (ccd) {
             hipLaunchKernelGGL((k_vertex_records<T>), dim3(grid_for(3 * n, 256), B), dim3(256), 0, s, (const T *)q.t0,
@@ -1650,13 +1748,17 @@ static ccc_status run_query(const Query &q) {
                     hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
                                        q.out_rows, q.out_cap, ctr);
                 else if (q.narrow == Narrow::CCD_COLLECT)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT>), dim3(gCcdC, B), dim3(ccd_threads<MODE_COLLECT>()), 0, s, items, capM, (const T *)tv,
-                                       (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
-                                       (T *)nullptr, (T *)nullptr, (int *)nullptr, ctr);
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT, false>), dim3(gCcdC, B), dim3(ccd_threads<MODE_COLLECT>()), 0, s, items, capM,
+                                       (const T *)tv, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
+                                       (T *)nullptr, (T *)nullptr, (int *)nullptr, (long long *)nullptr, ctr);
+                else if (q.det_acc)
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, true>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM,
+                                       (const T *)tv, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
+                                       (T *)q.dv, (T *)q.dx, q.cnt, q.det_acc, ctr);
                 else
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM, (const T *)tv,
-                                       (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
-                                       (T *)q.dv, (T *)q.dx, q.cnt, ctr);
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, false>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM,
+                                       (const T *)tv, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
+                                       (T *)q.dv, (T *)q.dx, q.cnt, (long long *)nullptr, ctr);
                 tm.end();
                 CCC_LAUNCH_CHECK("narrow phase");
                 hipLaunchKernelGGL(k_fold_items, dim3(1), dim3(64), 0, s, ctr, B);
@@ -1665,7 +1767,13 @@ static ccc_status run_query(const Query &q) {
             return CCC_OK;
         };
         if ((st = narrow_phase()) != CCC_OK) return st;
+        if (q.det_acc) {
+            hipLaunchKernelGGL((k_det_finalize<T>), dim3(grid_for(B * n * 3, 256)), dim3(256), 0, s, q.det_acc, B * n * 3,
+                               (T *)q.dv, (T *)q.dx);
+            CCC_LAUNCH_CHECK("k_det_finalize");
+        }
     }
+    if (q.nosync) return CCC_OK;
     CCC_HIP_TRY(hipMemcpyAsync(h.data(), ctr, sizeof(DevCounters) * B, hipMemcpyDeviceToHost, s));
     CCC_HIP_TRY(hipStreamSynchronize(s));
     bool truncated = false;
@@ -1685,6 +1793,67 @@ static ccc_status run_query(const Query &q) {
     if (warn_cand) out |= CCC_WARN_CANDIDATES;
     if (truncated) out |= CCC_WARN_TRUNCATED;
     return out;
+}
+
+// Workspace bytes of a resident CCD_IMPULSE pass (nosync, caller-owned counters and pair
+// list): the same buffers, sizes and 256-byte rounding as run_query's requests.
+static size_t resident_bytes(const Query &q) {
+    const int64_t B = q.B, n = q.n, nint = n > 1 ? n - 1 : 1;
+    const size_t esz = q.dtype == CCC_FLOAT64 ? 8 : 4;
+    auto r = [](size_t b) { return ((b ? b : 256) + 255) & ~(size_t)255; };
+    size_t tot = 0;
+    if (!q.ctr_ext) tot += r(sizeof(DevCounters) * B);
+    tot += r(4 * B * n) + 3 * r(4 * B * nint);
+    tot += r(sizeof(TNode) * B * nint) + r(sizeof(QNode) * B * nint) + r(sizeof(LeafRec) * B * n) + r(sizeof(Frontier) * B);
+    const unsigned gT = kFrontier * (unsigned)((grid_for(n, kTravThreads) + kXcds - 1) / kXcds * kXcds);
+    tot += r(sizeof(float) * B * (int64_t)(gT / kFrontier) * 6);
+    const int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
+    if (!q.pairs_ext) tot += r(sizeof(int2) * B * capK);
+    const int64_t budget = q.chunk_ext > 0 ? q.chunk_ext * B : q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
+    const int64_t chunk = std::max<int64_t>(std::min<int64_t>(capK, budget / B), 1);
+    tot += r(esz * B * n * 18) + r(sizeof(int64_t) * B * 15 * chunk);
+    return tot;
+}
+
+static Query resident_query(ccc_context *ctx, const ResidentPass &p, hipStream_t s) {
+    Query q{};
+    q.ctx = ctx;
+    q.bboxes = p.bboxes;
+    q.tree = p.tree;
+    q.t0 = p.t0;
+    q.t1 = p.t1;
+    q.masses = p.masses;
+    q.B = 1;
+    q.n = p.n;
+    q.dtype = p.dtype;
+    q.thr = 0.0;
+    q.cap_candidates = p.cap_pairs;
+    q.narrow = Narrow::CCD_IMPULSE;
+    q.dv = p.dv;
+    q.dx = p.dx;
+    q.cnt = p.cnt;
+    q.mesh_end = p.mesh_end;
+    q.flags = CCC_FLAG_FIXED_CAPACITY;
+    q.s = s;
+    q.nosync = true;
+    q.ctr_ext = p.ctr;
+    q.pairs_ext = p.pairs;
+    q.chunk_ext = p.chunk;
+    q.ccd_blocks_per_cu = p.ccd_blocks_per_cu;
+    q.stack_deep = p.stack_deep;
+    q.det_acc = p.det_acc;
+    return q;
+}
+
+size_t resident_pass_arena_bytes(ccc_context *ctx, const ResidentPass &p) { return resident_bytes(resident_query(ctx, p, nullptr)); }
+
+ccc_status resident_pass(ccc_context *ctx, const ResidentPass &p, void *arena, size_t arena_bytes, hipStream_t s) {
+    Query q = resident_query(ctx, p, s);
+    q.arena = arena;
+    q.arena_bytes = arena_bytes;
+    if (p.dtype == CCC_FLOAT64) return run_query<double>(q);
+    if (p.dtype == CCC_FLOAT32) return run_query<float>(q);
+    return bad_arg("dtype must be CCC_FLOAT32 or CCC_FLOAT64");
 }
 
 static ccc_status run(const Query &q) {

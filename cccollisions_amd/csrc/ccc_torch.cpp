@@ -477,9 +477,157 @@ std::vector<std::vector<int64_t>> last_counts() {
     return out;
 }
 
+// ---- resident simulations ------------------------------------------------------------------
+// Sim(x0, x1, faces, masses, ...) copies one simulation (or several meshes packed back to
+// back) onto the device; step() runs `iters` impulse passes on the current stream, in place
+// on the resident x1, as one replayed HIP graph.
+class PySim {
+  public:
+    PySim(at::Tensor x0, at::Tensor x1, at::Tensor faces, at::Tensor masses, std::vector<int64_t> mesh_offsets, int iters,
+          int max_candidates, bool deterministic, bool graph, int morton, int ccd_blocks_per_cu, int64_t pair_chunk) {
+        CHECK_INPUT(x0);
+        CHECK_INPUT(x1);
+        CHECK_INPUT(faces);
+        CHECK_INPUT(masses);
+        TORCH_CHECK(x0.dim() == 2 && x0.size(1) == 3, "x0 must have shape [V, 3]");
+        TORCH_CHECK(x1.sizes() == x0.sizes(), "x1 must have the shape of x0");
+        TORCH_CHECK(faces.dim() == 2 && faces.size(1) == 3 && faces.scalar_type() == at::kLong, "faces must be int64 [F, 3]");
+        TORCH_CHECK(masses.dim() == 2 && masses.size(0) == faces.size(0) && masses.size(1) == 3, "masses must have shape [F, 3]");
+        check_like(x1, x0, "x1");
+        check_like(masses, x0, "masses");
+        TORCH_CHECK(faces.device() == x0.device(), "faces must be on ", x0.device());
+        const at::DeviceGuard guard(x0.device());
+        opts_ = x0.options();
+        nv_ = x0.size(0);
+        nf_ = faces.size(0);
+        ccc_sim_opts o;
+        ccc_sim_default_opts(&o);
+        o.iters = iters;
+        o.max_candidates = max_candidates;
+        o.deterministic = deterministic ? 1 : 0;
+        o.graph = graph ? 1 : 0;
+        o.morton = morton;
+        o.ccd_blocks_per_cu = ccd_blocks_per_cu;
+        o.pair_chunk = pair_chunk;
+        ccc_status st = CCC_NOGIL(ccc_sim_create(context_for(x0), x0.data_ptr(), x1.data_ptr(), nv_, faces.data_ptr<int64_t>(),
+                                                 masses.data_ptr(), faces.size(0), mesh_offsets.empty() ? nullptr : mesh_offsets.data(),
+                                                 mesh_offsets.empty() ? 1 : (int64_t)mesh_offsets.size() - 1, dtype_of(x0), &o,
+                                                 stream_for(x0), &sim_));
+        TORCH_CHECK(st == CCC_OK, "Sim: creation failed: ", ccc_status_string(st), ": ", ccc_last_error());
+    }
+    ~PySim() {
+        if (sim_) ccc_sim_destroy(sim_);
+    }
+    void step(bool advance, c10::optional<at::Tensor> workspace) {
+        const at::DeviceGuard guard(opts_.device());
+        void *ws = nullptr;
+        size_t bytes = 0;
+        if (workspace && workspace->defined()) {
+            TORCH_CHECK(workspace->is_cuda() && workspace->is_contiguous(), "workspace must be a contiguous device tensor");
+            ws = workspace->data_ptr();
+            bytes = (size_t)workspace->nbytes();
+        }
+        auto stream = (void *)c10::hip::getCurrentHIPStream(opts_.device().index()).stream();
+        ccc_status st = CCC_NOGIL(ccc_sim_step(sim_, ws, bytes, advance ? 1 : 0, stream));
+        TORCH_CHECK(st == CCC_OK, "Sim.step failed: ", ccc_status_string(st), ": ", ccc_last_error());
+    }
+    void autotune(c10::optional<at::Tensor> workspace) {
+        const at::DeviceGuard guard(opts_.device());
+        void *ws = nullptr;
+        size_t bytes = 0;
+        if (workspace && workspace->defined()) {
+            ws = workspace->data_ptr();
+            bytes = (size_t)workspace->nbytes();
+        }
+        auto stream = (void *)c10::hip::getCurrentHIPStream(opts_.device().index()).stream();
+        ccc_status st = CCC_NOGIL(ccc_sim_autotune(sim_, ws, bytes, stream));
+        TORCH_CHECK(st == CCC_OK, "Sim.autotune failed: ", ccc_status_string(st), ": ", ccc_last_error());
+    }
+    // Copies of the resident positions (x0, x1), [V, 3].
+    std::vector<at::Tensor> positions() {
+        const at::DeviceGuard guard(opts_.device());
+        void *x0 = nullptr, *x1 = nullptr;
+        ccc_sim_state(sim_, &x0, &x1, nullptr, nullptr);
+        auto a = at::from_blob(x0, {nv_, 3}, opts_).clone();
+        auto b = at::from_blob(x1, {nv_, 3}, opts_).clone();
+        return {a, b};
+    }
+    // Overwrites the resident positions (e.g. with the simulator's next prediction).
+    void set_positions(at::Tensor x0, at::Tensor x1) {
+        CHECK_INPUT(x0);
+        CHECK_INPUT(x1);
+        TORCH_CHECK(x0.scalar_type() == opts_.dtype().toScalarType() && x1.scalar_type() == x0.scalar_type(), "dtype mismatch");
+        TORCH_CHECK(x0.numel() == nv_ * 3 && x1.numel() == nv_ * 3, "positions must have shape [V, 3]");
+        const at::DeviceGuard guard(opts_.device());
+        void *d0 = nullptr, *d1 = nullptr;
+        ccc_sim_state(sim_, &d0, &d1, nullptr, nullptr);
+        at::from_blob(d0, {nv_, 3}, opts_).copy_(x0.reshape({nv_, 3}));
+        at::from_blob(d1, {nv_, 3}, opts_).copy_(x1.reshape({nv_, 3}));
+    }
+    // Copies of the last pass's impulse outputs (dv, dx [F, 3, 3], cnt [F, 3]).
+    std::vector<at::Tensor> last_impulses() {
+        const at::DeviceGuard guard(opts_.device());
+        void *dv = nullptr, *dx = nullptr;
+        int32_t *cnt = nullptr;
+        ccc_sim_impulses(sim_, &dv, &dx, &cnt);
+        const int64_t nf = nf_;
+        return {at::from_blob(dv, {nf, 3, 3}, opts_).clone(), at::from_blob(dx, {nf, 3, 3}, opts_).clone(),
+                at::from_blob(cnt, {nf, 3}, opts_.dtype(at::kInt)).clone()};
+    }
+    std::vector<std::vector<int64_t>> counts() const {
+        std::vector<ccc_counts> c(64);
+        const int n = ccc_sim_step_counts(sim_, c.data(), (int32_t)c.size());
+        std::vector<std::vector<int64_t>> out;
+        for (int i = 0; i < std::min(n, 64); ++i) out.push_back({c[i].candidates, c[i].subprims, c[i].roots, c[i].collisions});
+        return out;
+    }
+    py::dict config() const {
+        ccc_sim_opts o;
+        ccc_sim_get_opts(sim_, &o);
+        int64_t cap = 0;
+        ccc_sim_state(sim_, nullptr, nullptr, nullptr, &cap);
+        py::dict d;
+        d["iters"] = o.iters;
+        d["deterministic"] = (bool)o.deterministic;
+        d["graph"] = (bool)o.graph;
+        d["morton"] = o.morton;
+        d["ccd_blocks_per_cu"] = o.ccd_blocks_per_cu;
+        d["pair_chunk"] = o.pair_chunk;
+        d["candidate_capacity"] = cap;
+        return d;
+    }
+    std::vector<double> tuning_times() const {
+        double ms[8];
+        const int n = ccc_sim_tuning_times(sim_, ms, 8);
+        return std::vector<double>(ms, ms + std::min(n, 8));
+    }
+    int64_t footprint() const { return (int64_t)ccc_sim_footprint(sim_); }
+    int64_t workspace_bytes() const { return (int64_t)ccc_sim_workspace_bytes(sim_); }
+
+  private:
+    ccc_sim *sim_ = nullptr;
+    at::TensorOptions opts_;
+    int64_t nv_ = 0, nf_ = 0;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_cccollisions, m) {
+    py::class_<PySim>(m, "Sim", "A simulation (or packed meshes) resident on the device; step() runs `iters` impulse passes")
+        .def(py::init<at::Tensor, at::Tensor, at::Tensor, at::Tensor, std::vector<int64_t>, int, int, bool, bool, int, int, int64_t>(),
+             py::arg("x0"), py::arg("x1"), py::arg("faces"), py::arg("masses"), py::arg("mesh_offsets") = std::vector<int64_t>{},
+             py::arg("iters") = 10, py::arg("max_candidates") = 64, py::arg("deterministic") = false, py::arg("graph") = true,
+             py::arg("morton") = 0, py::arg("ccd_blocks_per_cu") = 128, py::arg("pair_chunk") = 0)
+        .def("step", &PySim::step, py::arg("advance") = false, py::arg("workspace") = py::none())
+        .def("autotune", &PySim::autotune, py::arg("workspace") = py::none())
+        .def("positions", &PySim::positions)
+        .def("set_positions", &PySim::set_positions, py::arg("x0"), py::arg("x1"))
+        .def("last_impulses", &PySim::last_impulses)
+        .def("counts", &PySim::counts)
+        .def("config", &PySim::config)
+        .def("tuning_times", &PySim::tuning_times)
+        .def("footprint", &PySim::footprint)
+        .def("workspace_bytes", &PySim::workspace_bytes);
     m.doc() = "MI355X-native CCCollisions entry points (drop-in for the `cccollisions` extension)";
     m.def("bvh", &bvh_forward, "BVH over static triangles", py::arg("triangles"));
     m.def("bvh_motion", &bvh_forward_motion, "BVH over swept triangles", py::arg("triangles"), py::arg("triangles_next"));

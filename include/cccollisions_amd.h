@@ -247,6 +247,67 @@ ccc_status ccc_compute_impulses_noalloc(ccc_context *ctx, const void *bboxes, co
                                         int32_t max_candidates, int32_t max_collisions, void *dv, void *dx,
                                         int32_t *cnt, ccc_counts *counts, uint32_t flags, void *stream);
 
+/* ---- Resident simulations (no counterpart in the reference) --------------------------------
+ * The reference is stateless: its caller (a cloth simulator, not part of the reference) runs
+ * bvh_motion + compute_impulses and applies the corrections itself, with a host
+ * synchronisation after every launch (src/flags.h:63-77).  A ccc_sim keeps one simulation
+ * resident on the device -- positions x0 (start of step) and x1 (predicted end of step)
+ * [V, 3], faces [F, 3] int64, corner masses [F, 3], triangle soups, BVH, candidate-pair
+ * list, impulse outputs and a vertex -> corner index -- and runs whole steps of `iters`
+ * impulse passes; a pass is bvh_motion (src/build_bvh.cu:570-648) + compute_impulses
+ * (src/collisions_impulses.cu:1069-1397) + the caller's update: every vertex of x1 moves by
+ * the mean of the finite position corrections dx of its triangle corners (corners summed in
+ * ascending order; a corner with a non-finite dx is skipped).  Several meshes of different
+ * sizes may be packed into one simulation (faces index one vertex array; mesh m owns faces
+ * [offsets[m], offsets[m+1])): pairs never cross meshes, one BVH and one traversal cover
+ * them all.  A step neither allocates nor waits on the host until its end: it is captured
+ * once as a HIP graph (opts.graph) per workspace arena and replayed.  At the end of a step
+ * the per-pass counters are read; a pass that found more candidate pairs than the list
+ * holds, or whose traversal needed the deep stack, grows the list and the step runs again
+ * from the saved x1.  All calls on one simulation must be serialised by the caller. */
+typedef struct ccc_sim ccc_sim;
+typedef struct {
+    int32_t iters;              /* impulse passes per step (default 10)                          */
+    int32_t max_candidates;     /* initial candidate capacity per triangle (default 64; grows)  */
+    int32_t deterministic;      /* order-independent fixed-point impulse sums: every step gives
+                                   the same bits, alone or beside other work (default 0)      */
+    int32_t graph;              /* capture each step as one HIP graph (default 1)               */
+    int32_t morton;             /* CCC_MORTON_* of the step's trees (one-mesh simulations)      */
+    int32_t ccd_blocks_per_cu;  /* grid cap of the narrow-phase kernel (default 128)             */
+    int64_t pair_chunk;         /* candidate pairs per narrow-phase chunk (0: 2^25)            */
+} ccc_sim_opts;
+void ccc_sim_default_opts(ccc_sim_opts *opts);
+/* x0, x1: device [V, 3]; faces: device [F, 3] int64; masses: device [F, 3]; all copied into
+ * the simulation.  mesh_face_offsets: HOST array of num_meshes + 1 entries, or NULL for one
+ * mesh.  The call synchronises `stream`. */
+ccc_status ccc_sim_create(ccc_context *ctx, const void *x0, const void *x1, int64_t num_vertices, const int64_t *faces,
+                          const void *masses, int64_t num_faces, const int64_t *mesh_face_offsets, int64_t num_meshes,
+                          int32_t dtype, const ccc_sim_opts *opts, void *stream, ccc_sim **out);
+void ccc_sim_destroy(ccc_sim *sim);
+/* Bytes of device workspace one step needs (at the current candidate capacity). */
+size_t ccc_sim_workspace_bytes(const ccc_sim *sim);
+/* One step on `stream` (synchronised before returning).  workspace: NULL for the
+ * simulation's own (allocated when first needed), or a caller-owned device arena of at least
+ * ccc_sim_workspace_bytes bytes (e.g. one per serving slot; CCC_ERR_NO_MEMORY if smaller).
+ * advance != 0: afterwards x0 <- x1 and x1 <- x1 + (x1 - x0) (the next step's prediction). */
+ccc_status ccc_sim_step(ccc_sim *sim, void *workspace, size_t workspace_bytes, int32_t advance, void *stream);
+/* Times steps under candidate launch configurations (narrow-phase grid 64 / 128 / 256 blocks
+ * per CU; for one-mesh simulations also the reference's and the 30-bit fitted Morton tree),
+ * keeps the fastest and remembers it for the context's simulations of the same size bucket
+ * (then no timing).  x1 is left as it was. */
+ccc_status ccc_sim_autotune(ccc_sim *sim, void *workspace, size_t workspace_bytes, void *stream);
+ccc_status ccc_sim_get_opts(const ccc_sim *sim, ccc_sim_opts *opts);
+/* Milliseconds per step of each configuration the last autotune timed; returns their number. */
+int32_t ccc_sim_tuning_times(const ccc_sim *sim, double *ms, int32_t max);
+/* Device addresses of the resident positions ([V, 3] of the simulation's dtype). */
+ccc_status ccc_sim_state(const ccc_sim *sim, void **x0, void **x1, int64_t *num_vertices, int64_t *candidate_capacity);
+/* Device addresses of the last pass's impulse outputs: dv, dx [F, 3, 3], cnt [F, 3] int32. */
+ccc_status ccc_sim_impulses(const ccc_sim *sim, void **dv, void **dx, int32_t **cnt);
+/* Stage counts of each pass of the last step; returns the number of passes. */
+int32_t ccc_sim_step_counts(const ccc_sim *sim, ccc_counts *per_pass, int32_t max_passes);
+/* Device bytes the simulation holds (resident state + own workspace). */
+size_t ccc_sim_footprint(const ccc_sim *sim);
+
 /* Replaces collision_impulses_from_bbox_tree_partial -- src/collisions_impulses.cu:1402-1734
  * (binding src/cccollisions.cpp:336-370). */
 ccc_status ccc_compute_impulses_partial(ccc_context *ctx, const void *bboxes, const int64_t *tree,
