@@ -823,6 +823,76 @@ __device__ __forceinline__ void scatter_impulse(const MTet<T> &m, int type, T ro
     }
 }
 
+// ---- order-independent impulse sums (deterministic mode) ----------------------------------
+// A finite value v with |v| < 2^45 is held as the integer trunc(|v| * 2^80) with v's sign,
+// split into three 42-bit limbs; every limb of every contribution is added with a 64-bit
+// integer atomic, which is exact and associative, so the sums do not depend on the order of
+// the additions (2^21 additions per limb fit in 64 bits).  Values that cannot be held set
+// flag bits in a fourth word instead: 1 NaN, 2 +inf, 4 -inf, 8 out of range; the result
+// is then what an fp64 sum gives (NaN, or the infinity) and NaN when out of range.
+// Per triangle corner: 6 components (dv xyz, dx xyz) x 4 words.
+constexpr int kDetWords = kDetWordsPerCorner;
+constexpr double kTwo42 = 4398046511104.0, kTwo45 = 35184372088832.0, kTwo80 = 1208925819614629174706176.0;
+
+// The three limbs of v (returns 0), or its flag bit when v cannot be held.
+__device__ __forceinline__ unsigned det_split(double v, long long &l0, long long &l1, long long &l2) {
+    if (!(fabs(v) < kTwo45))       // NaN fails this test too
+        return v != v ? 1u : v == (double)INFINITY ? 2u : v == -(double)INFINITY ? 4u : 8u;
+    const double sc = fabs(v) * kTwo80;                  // exact: a power-of-two scaling, < 2^125
+    const double hi = floor(sc / (kTwo42 * kTwo42));    // < 2^41; exact
+    const double r1 = sc - hi * (kTwo42 * kTwo42);      // the bits of sc below 2^84: exact
+    const double mid = floor(r1 / kTwo42);              // < 2^42
+    const double lo = floor(r1 - mid * kTwo42);         // < 2^42, fraction truncated
+    l0 = (long long)lo;
+    l1 = (long long)mid;
+    l2 = (long long)hi;
+    if (v < 0) { l0 = -l0; l1 = -l1; l2 = -l2; }
+    return 0u;
+}
+
+__device__ __forceinline__ void det_add1(long long *a, double v) {
+    long long l0, l1, l2;
+    unsigned long long *u = reinterpret_cast<unsigned long long *>(a);
+    const unsigned f = det_split(v, l0, l1, l2);
+    if (f) {
+        atomicOr(u + 3, (unsigned long long)f);
+        return;
+    }
+    if (l0) atomicAdd(u + 0, (unsigned long long)l0);
+    if (l1) atomicAdd(u + 1, (unsigned long long)l1);
+    if (l2) atomicAdd(u + 2, (unsigned long long)l2);
+}
+
+// Limbs -> dv / dx of every corner: carries normalise the two low limbs into [0, 2^42),
+// then hi * 2^84 + mid * 2^42 + lo is formed in double in that order and scaled by 2^-80.
+template <typename T>
+__global__ void k_det_finalize(const long long *__restrict__ acc, int64_t corners, T *__restrict__ dv, T *__restrict__ dx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= corners) return;
+    const long long *a = acc + kDetWords * c;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        long long l0 = a[4 * k], l1 = a[4 * k + 1], l2 = a[4 * k + 2];
+        const unsigned long long f = (unsigned long long)a[4 * k + 3];
+        double v;
+        if (f & 9ull) v = (double)NAN;
+        else if ((f & 6ull) == 6ull) v = (double)NAN;
+        else if (f & 2ull) v = (double)INFINITY;
+        else if (f & 4ull) v = -(double)INFINITY;
+        else {
+            const long long c0 = l0 >> 42;
+            l0 -= c0 * (1ll << 42);
+            l1 += c0;
+            const long long c1 = l1 >> 42;
+            l1 -= c1 * (1ll << 42);
+            l2 += c1;
+            v = ((double)l2 * (kTwo42 * kTwo42) + (double)l1 * kTwo42 + (double)l0) / kTwo80;
+        }
+        T *o = k < 3 ? dv + 3 * c + k : dx + 3 * c + (k - 3);
+        *o = (T)v;
+    }
+}
+
 // Per-block sums of corner impulses: open addressing on the corner key 3 * triangle +
 // corner, kImpSlots slots (a power of two), linear probing up to kImpProbe slots; a
 // contribution that finds no slot goes to the global arrays directly.
@@ -900,76 +970,79 @@ template <typename T> struct ImpTable {
 };
 static_assert(kImpSlots == 1 << kImpBits, "the slot count is a power of two");
 
-// ---- order-independent impulse sums (deterministic mode) ----------------------------------
-// A finite value v with |v| < 2^45 is held as the integer trunc(|v| * 2^80) with v's sign,
-// split into three 42-bit limbs; every limb of every contribution is added with a 64-bit
-// integer atomic, which is exact and associative, so the sums do not depend on the order of
-// the additions (2^21 additions per limb fit in 64 bits).  Values that cannot be held set
-// flag bits in a fourth word instead: 1 NaN, 2 +inf, 4 -inf, 8 out of range; the result
-// is then what an fp64 sum gives (NaN, or the infinity) and NaN when out of range.
-// Per triangle corner: 6 components (dv xyz, dx xyz) x 4 words.
-constexpr int kDetWords = kDetWordsPerCorner;
-constexpr double kTwo42 = 4398046511104.0, kTwo45 = 35184372088832.0, kTwo80 = 1208925819614629174706176.0;
+// The deterministic mode's per-block table: the same slots and probing as ImpTable, holding
+// the two low limbs of each component as 64-bit integers (LDS integer atomics: exact, so the
+// table's merge is order-independent too).  High limbs and flags, which cloth-scale
+// corrections rarely have, go to the global limbs directly; each occupied slot then adds its
+// two limbs per component and its count with global atomics.
+struct DetTable {
+    int *key;
+    int *count;
+    unsigned long long *acc;    // [slot][component][lo, mid]
 
-__device__ __forceinline__ void det_add1(long long *a, double v) {
-    if (!(fabs(v) < kTwo45)) {       // NaN fails this test too
-        const unsigned long long f = v != v ? 1ull : v == (double)INFINITY ? 2ull : v == -(double)INFINITY ? 4ull : 8ull;
-        atomicOr(reinterpret_cast<unsigned long long *>(a + 3), f);
-        return;
+    static constexpr size_t bytes() { return (size_t)kImpSlots * (2 * sizeof(int) + 12 * sizeof(unsigned long long)); }
+    __device__ static DetTable over(unsigned char *raw) {
+        DetTable t;
+        t.acc = reinterpret_cast<unsigned long long *>(raw);
+        t.key = reinterpret_cast<int *>(raw + (size_t)kImpSlots * 12 * sizeof(unsigned long long));
+        t.count = t.key + kImpSlots;
+        return t;
     }
-    const double sc = fabs(v) * kTwo80;                  // exact: a power-of-two scaling, < 2^125
-    const double hi = floor(sc / (kTwo42 * kTwo42));    // < 2^41; exact
-    const double r1 = sc - hi * (kTwo42 * kTwo42);      // the bits of sc below 2^84: exact
-    const double mid = floor(r1 / kTwo42);              // < 2^42
-    const double lo = floor(r1 - mid * kTwo42);         // < 2^42, fraction truncated
-    long long l0 = (long long)lo, l1 = (long long)mid, l2 = (long long)hi;
-    if (v < 0) { l0 = -l0; l1 = -l1; l2 = -l2; }
-    unsigned long long *u = reinterpret_cast<unsigned long long *>(a);
-    if (l0) atomicAdd(u + 0, (unsigned long long)l0);
-    if (l1) atomicAdd(u + 1, (unsigned long long)l1);
-    if (l2) atomicAdd(u + 2, (unsigned long long)l2);
-}
-
-template <typename T>
-__device__ __forceinline__ void det_add_corner(long long *acc, int64_t c, const V3<T> &imp, const V3<T> &imd) {
-    long long *a = acc + kDetWords * c;
-    det_add1(a + 0, (double)imp.x);
-    det_add1(a + 4, (double)imp.y);
-    det_add1(a + 8, (double)imp.z);
-    det_add1(a + 12, (double)imd.x);
-    det_add1(a + 16, (double)imd.y);
-    det_add1(a + 20, (double)imd.z);
-}
-
-// Limbs -> dv / dx of every corner: carries normalise the two low limbs into [0, 2^42),
-// then hi * 2^84 + mid * 2^42 + lo is formed in double in that order and scaled by 2^-80.
-template <typename T>
-__global__ void k_det_finalize(const long long *__restrict__ acc, int64_t corners, T *__restrict__ dv, T *__restrict__ dx) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= corners) return;
-    const long long *a = acc + kDetWords * c;
+    __device__ void clear(int tid) {
+        for (int s = tid; s < kImpSlots; s += kCcdThreads) {
+            key[s] = -1;
+            count[s] = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        long long l0 = a[4 * k], l1 = a[4 * k + 1], l2 = a[4 * k + 2];
-        const unsigned long long f = (unsigned long long)a[4 * k + 3];
-        double v;
-        if (f & 9ull) v = (double)NAN;
-        else if ((f & 6ull) == 6ull) v = (double)NAN;
-        else if (f & 2ull) v = (double)INFINITY;
-        else if (f & 4ull) v = -(double)INFINITY;
-        else {
-            const long long c0 = l0 >> 42;
-            l0 -= c0 * (1ll << 42);
-            l1 += c0;
-            const long long c1 = l1 >> 42;
-            l1 -= c1 * (1ll << 42);
-            l2 += c1;
-            v = ((double)l2 * (kTwo42 * kTwo42) + (double)l1 * kTwo42 + (double)l0) / kTwo80;
+            for (int c = 0; c < 12; ++c) acc[12 * s + c] = 0ull;
         }
-        T *o = k < 3 ? dv + 3 * c + k : dx + 3 * c + (k - 3);
-        *o = (T)v;
     }
-}
+    __device__ int slot(int k) {
+        int h = (int)(((unsigned)k * 2654435761u) >> (32 - kImpBits)) & (kImpSlots - 1);
+        for (int p = 0; p < kImpProbe; ++p) {
+            const int old = atomicCAS(&key[h], -1, k);
+            if (old == -1 || old == k) return h;
+            h = (h + 1) & (kImpSlots - 1);
+        }
+        return -1;
+    }
+    __device__ void add(int64_t c, const double v[6], long long *det, int *cnt) {
+        const int s = slot((int)c);
+        unsigned long long *g = reinterpret_cast<unsigned long long *>(det + kDetWords * c);
+        if (s < 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) det_add1(det + kDetWords * c + 4 * k, v[k]);
+            atomicAdd(cnt + c, 1);
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            long long l0, l1, l2;
+            const unsigned f = det_split(v[k], l0, l1, l2);
+            if (f) {
+                atomicOr(g + 4 * k + 3, (unsigned long long)f);
+                continue;
+            }
+            if (l0) atomicAdd(&acc[12 * s + 2 * k], (unsigned long long)l0);
+            if (l1) atomicAdd(&acc[12 * s + 2 * k + 1], (unsigned long long)l1);
+            if (l2) atomicAdd(g + 4 * k + 2, (unsigned long long)l2);
+        }
+        atomicAdd(&count[s], 1);
+    }
+    __device__ void drain(int tid, long long *det, int *cnt) {
+        for (int s = tid; s < kImpSlots; s += kCcdThreads) {
+            const int k = key[s];
+            if (k < 0) continue;
+            unsigned long long *g = reinterpret_cast<unsigned long long *>(det + kDetWords * (int64_t)k);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const unsigned long long lo = acc[12 * s + 2 * c], mid = acc[12 * s + 2 * c + 1];
+                if (lo) atomicAdd(g + 4 * c, lo);
+                if (mid) atomicAdd(g + 4 * c + 1, mid);
+            }
+            atomicAdd(cnt + k, count[s]);
+        }
+    }
+};
 
 // Hides a pointer's provenance from the optimiser so that loads through it are not
 // merged with earlier loads of the same addresses.
@@ -1033,7 +1106,8 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     // the records (phases A-C) and the root list (phases B-C) share their LDS with the
     // impulse table of phase D
     constexpr size_t kSolveBytes = sizeof(SolveRec) * NT + (sizeof(T) + sizeof(unsigned short)) * kRootCapT;
-    constexpr size_t kScatterBytes = ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush;
+    constexpr size_t kTableBytes = DET ? DetTable::bytes() : ImpTable<T>::bytes();
+    constexpr size_t kScatterBytes = kTableBytes + sizeof(ImpulseParams<T>) * kFlush;
     __shared__ __align__(16) unsigned char s_raw[kSolveBytes > kScatterBytes ? kSolveBytes : kScatterBytes];
     SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
     T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * NT);
@@ -1066,11 +1140,11 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     // Staged collisions are scattered kFlush at a time: their corner impulses are first
     // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
     // dx / cnt with one set of global atomics.
-    static_assert(ImpTable<T>::bytes() + sizeof(ImpulseParams<T>) * kFlush <= sizeof(s_raw),
-                  "the impulse table and parameters fit the phase A-C buffers");
+    static_assert(kScatterBytes <= sizeof(s_raw), "the impulse table and parameters fit the phase A-C buffers");
     static_assert(kFlush <= NT && (4 * kFlush) % NT == 0, "whole rounds of corners");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
-    ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + ImpTable<T>::bytes());
+    DetTable dtab = DetTable::over(s_raw);
+    ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + kTableBytes);
     // Scatters staged collisions [0, f) in chunks of kFlush: one thread per collision
     // computes its impulse parameters, then one thread per tetrahedron corner adds that
     // corner's share to the LDS table (the corner step holds few registers, so the
@@ -1078,7 +1152,8 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     auto flush = [&](int f) {
         for (int base = 0; base < f; base += kFlush) {
             const int nf = f - base < kFlush ? f - base : kFlush;
-            if (!DET) tab.clear(tid);
+            if (DET) dtab.clear(tid);
+            else tab.clear(tid);
             if (tid < nf) {
                 int64_t x, y;
                 int type;
@@ -1111,15 +1186,16 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
                     V3<T> imp, imd;
                     corner_impulse(s_imp[jc], type, k, dvk, imp, imd);
                     if (DET) {
-                        det_add_corner(det, c, imp, imd);
-                        atomicAdd(cnt + c, 1);
+                        const double v[6] = {(double)imp.x, (double)imp.y, (double)imp.z, (double)imd.x, (double)imd.y, (double)imd.z};
+                        dtab.add(c, v, det, cnt);
                     } else {
                         tab.add(c, imp, imd, dv, dx, cnt);
                     }
                 }
             }
             __syncthreads();
-            if (!DET) tab.drain(tid, dv, dx, cnt);
+            if (DET) dtab.drain(tid, det, cnt);
+            else tab.drain(tid, dv, dx, cnt);
             __syncthreads();
         }
     };
