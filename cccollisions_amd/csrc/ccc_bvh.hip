@@ -436,7 +436,7 @@ template <typename T>
 static ccc_status build(ccc_context *ctx, const T *t0, const T *t1, int64_t B, int64_t n, int morton, T *bboxes, int64_t *tree,
                         hipStream_t s, void *arena, size_t arena_bytes) {
     Workspace ws(ctx, s, arena, arena_bytes);
-    KernelTimer tm(arena ? nullptr : ctx, s);
+    KernelTimer tm(ctx, s);
     const int64_t total = B * n;
     const int nb = (int)((n + kSortTile - 1) / kSortTile);
     const bool fit = morton != CCC_MORTON_COMPAT;

@@ -1613,7 +1613,8 @@ static ccc_status run_query(const Query &q) {
     const int64_t B = q.B, n = q.n, rows = 2 * n - 1;
     const int64_t nint = n > 1 ? n - 1 : 1;
     Workspace ws(q.ctx, s, q.arena, q.arena_bytes);
-    KernelTimer tm(q.arena ? nullptr : q.ctx, s);   // no event objects in arena mode
+    // no event objects in the allocation-free entry point (arena without resident mode)
+    KernelTimer tm(q.arena && !q.nosync ? nullptr : q.ctx, s);
     DevCounters *ctr = q.ctr_ext ? q.ctr_ext : ws.get_as<DevCounters>(B);
     int *claims = ws.get_as<int>(B * n);
     int *climb = ws.get_as<int>(B * nint);

@@ -193,25 +193,27 @@ static size_t arena_need(const ccc_sim *s) {
 template <typename T>
 static ccc_status enqueue_step(ccc_sim *s, void *arena, size_t bytes, hipStream_t st) {
     const int64_t corners = 3 * s->nf, coords = 3 * s->nv;
+    KernelTimer tm(s->ctx, st);     // records only while the context profiles (eager steps)
     hipLaunchKernelGGL((k_copy<T>), dim3(stride_grid(coords)), dim3(256), 0, st, (T *)s->x1_save, (const T *)s->x1, coords);
     hipLaunchKernelGGL((k_gather_soup<T>), dim3(grid_for(corners, 256)), dim3(256), 0, st, (const T *)s->x0, s->faces, corners,
                        (T *)s->t0);
     CCC_LAUNCH_CHECK("step preparation");
     const int morton = s->meshes > 1 ? CCC_MORTON_COMPAT : s->o.morton;
     for (int it = 0; it < s->o.iters; ++it) {
-        hipLaunchKernelGGL((k_gather_soup<T>), dim3(grid_for(corners, 256)), dim3(256), 0, st, (const T *)s->x1, s->faces, corners,
-                           (T *)s->t1);
+        CCC_TIMED(tm, "sim_gather", hipLaunchKernelGGL((k_gather_soup<T>), dim3(grid_for(corners, 256)), dim3(256), 0, st,
+                                                       (const T *)s->x1, s->faces, corners, (T *)s->t1));
         CCC_LAUNCH_CHECK("k_gather_soup");
         ccc_status r = build_bvh_device(s->ctx, s->t0, s->t1, 1, s->nf, s->dtype, morton, s->bboxes, s->tree, st, arena, bytes);
         if (r != CCC_OK) return r;
         r = resident_pass(s->ctx, pass_of(s, it), arena, bytes, st);
         if (r != CCC_OK) return r;
-        hipLaunchKernelGGL((k_apply_corrections<T>), dim3(grid_for(s->nv, 256)), dim3(256), 0, st, (T *)s->x1, (const T *)s->dx,
-                           s->cnt, s->csr_off, s->csr_corner, s->nv);
+        CCC_TIMED(tm, "sim_update", hipLaunchKernelGGL((k_apply_corrections<T>), dim3(grid_for(s->nv, 256)), dim3(256), 0, st,
+                                                       (T *)s->x1, (const T *)s->dx, s->cnt, s->csr_off, s->csr_corner, s->nv));
         CCC_LAUNCH_CHECK("k_apply_corrections");
     }
     return CCC_OK;
 }
+
 
 static ccc_status enqueue(ccc_sim *s, void *arena, size_t bytes, hipStream_t st) {
     return s->dtype == CCC_FLOAT64 ? enqueue_step<double>(s, arena, bytes, st) : enqueue_step<float>(s, arena, bytes, st);
@@ -219,7 +221,8 @@ static ccc_status enqueue(ccc_sim *s, void *arena, size_t bytes, hipStream_t st)
 
 // Launches one step on `st` (through the graph cached for `arena`, capturing it first).
 static ccc_status launch_step(ccc_sim *s, void *arena, size_t bytes, hipStream_t st) {
-    if (!s->o.graph) return enqueue(s, arena, bytes, st);
+    // a profiled step times its kernels with events and runs eagerly
+    if (!s->o.graph || s->ctx->profiling) return enqueue(s, arena, bytes, st);
     auto it = s->graphs.find(arena);
     if (it == s->graphs.end()) {
         CCC_HIP_TRY(hipStreamBeginCapture(s->cs, hipStreamCaptureModeThreadLocal));
@@ -529,6 +532,8 @@ ccc_status ccc_sim_autotune(ccc_sim *s, void *workspace, size_t workspace_bytes,
             drop_graphs(s);
             s->o.ccd_blocks_per_cu = grids[gi];
             s->o.morton = mortons[mi];
+            // with graphs the first run also captures; steps longer than 50 ms are timed once
+            // (capture is then a small part), shorter ones once more after the capture
             double ms = 1e300;
             for (int rep = 0; rep < 2 && r == CCC_OK; ++rep) {
                 if (reset() != hipSuccess) r = CCC_ERR_HIP;
@@ -536,7 +541,8 @@ ccc_status ccc_sim_autotune(ccc_sim *s, void *workspace, size_t workspace_bytes,
                 const auto t0 = std::chrono::steady_clock::now();
                 if (r == CCC_OK) r = run_step(s, workspace, workspace_bytes, st);
                 const double e = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                if (rep > 0 || !s->o.graph) ms = std::min(ms, e);    // with graphs, the first run captures
+                if (rep > 0 || !s->o.graph || e > 50.0) ms = std::min(ms, e);
+                if (rep == 0 && e > 50.0) break;
             }
             if (s->tuned_n < 8) s->tuned_ms[s->tuned_n++] = ms;
             if (r == CCC_OK && ms < best_ms) {

@@ -1,13 +1,19 @@
 """Continuous-batching request loop for CCD + impulse steps of many resident simulations.
 
-Requests name a resident simulation; each asks for one CCD + impulse step (`iters` impulse
-passes: bvh_motion + compute_impulses + vertex update) on that simulation's current
-positions.  Worker slots (one host thread and one HIP stream each) take batches of pending
-requests as they free up: requests are admitted in arrival order while the batch's
-estimated HBM footprint stays within the slot's budget and no other slot is working on the
-same simulation.  A batch runs through the packed entry point (one BVH / traversal /
-narrow phase over all its meshes), so requests that arrive while other batches are on the
-GPU join the next free slot and their work overlaps on the device with the running ones.
+Every simulation lives on the device as a `cccollisions_amd.Sim`: its positions, faces,
+masses, triangle soups, BVH and candidate-pair list stay resident between requests.  A
+request asks for one step of one simulation -- `iters` impulse passes (bvh_motion +
+compute_impulses + vertex update) replayed as one HIP graph -- after which the simulation
+advances (x0 <- the corrected positions, x1 <- a constant-velocity prediction), so that the
+next request continues from there.  Worker slots (one host thread, one HIP stream and one
+device workspace each) take batches of pending requests as they free up: requests are
+admitted in arrival order while the batch's working set (the steps' workspace bytes) stays
+within the slot's workspace and no other slot holds a request of the same simulation.
+Requests that arrive while other slots' steps run are admitted as soon as a slot frees up,
+and the slots' work overlaps on the device.  The server checks at start that the resident
+simulations plus the slots' workspaces fit its HBM budget.  With deterministic impulse sums
+(`Sim(..., deterministic=True)`, the default here) a simulation's states are the same bits
+whether its requests ran alone or beside others.
 """
 from __future__ import annotations
 
@@ -15,7 +21,7 @@ import collections
 import dataclasses
 import threading
 import time
-from typing import Callable, Dict, List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
@@ -35,17 +41,16 @@ class Request:
 
 
 def footprint_bytes(num_triangles: int, max_candidates: int, itemsize: int = 8) -> int:
-    """HBM a one-mesh step needs beyond the resident state: the two soups, the BVH and its
-    traversal nodes and leaf records, the candidate list at capacity, one chunk of
-    sub-primitive items, the narrow phase's vertex records and the impulse outputs."""
+    """Estimated device bytes of one resident simulation of a folded-cloth-like mesh
+    (V ~ N / 2 vertices) with `max_candidates` candidate pairs per triangle, plus the workspace
+    of one of its steps (tree preparation, vertex records and one chunk of sub-primitive
+    items); an estimate for planning before the simulations exist -- a created `Sim` reports
+    its own `footprint()` and `workspace_bytes()`."""
     n = num_triangles
-    soups = 2 * n * 9 * itemsize
-    bvh = (2 * n - 1) * (6 * itemsize + 4 * 8) + (n - 1) * 64 + n * 32
+    resident = (3 * (n // 2 + 1) * 3 + 4 * 9 * n + 3 * n) * itemsize + (2 * n - 1) * (6 * itemsize + 32) + 36 * n
     pairs = max_candidates * n * 8
-    items = 15 * min(max_candidates * n, 1 << 25) * 8
-    outs = n * (18 * itemsize + 12)
-    records = n * 18 * itemsize        # per-corner [t0 | t1] vertex records of the narrow phase
-    return soups + bvh + pairs + items + outs + records
+    workspace = n * (300 + 18 * itemsize) + 15 * min(max_candidates * n, 1 << 25) * 8
+    return resident + pairs + workspace
 
 
 class Scheduler:
@@ -104,67 +109,63 @@ class Scheduler:
 
 
 class Simulation:
-    """Resident state of one simulation on the device: faces, positions at the start of
-    the step (x0), predicted end-of-step positions (x1), per-corner masses, and t0 = x0
-    gathered per triangle (fixed for the step)."""
+    """One resident simulation: a `cccollisions_amd.Sim` over its faces, start-of-step
+    positions x0, predicted end-of-step positions x1 and per-corner masses."""
 
-    def __init__(self, torch, faces: np.ndarray, x0: np.ndarray, x1: np.ndarray, corner_masses: np.ndarray,
-                 device="cuda:0"):
+    def __init__(self, ccc, torch, faces: np.ndarray, x0: np.ndarray, x1: np.ndarray, corner_masses: np.ndarray,
+                 iters: int = 10, max_candidates: int = 512, deterministic: bool = True, device="cuda:0"):
         dev = torch.device(device)
-        self.faces = torch.from_numpy(np.ascontiguousarray(faces)).to(dev)
-        self.x0 = torch.from_numpy(np.ascontiguousarray(x0)).to(dev)
-        self.x1 = torch.from_numpy(np.ascontiguousarray(x1)).to(dev)
-        self.masses = torch.from_numpy(np.ascontiguousarray(corner_masses)).to(dev)
-        self.t0 = self.x0[self.faces].contiguous()
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        self.sim = ccc.Sim(t(x0), t(x1), t(faces), t(corner_masses), iters=iters, max_candidates=max_candidates,
+                           deterministic=deterministic)
         self.num_triangles = int(faces.shape[0])
         self.num_vertices = int(x0.shape[0])
+        self.steps = 0
 
+    def footprint(self) -> int:
+        return int(self.sim.footprint())
 
-def impulse_step(ccc, torch, sims: Sequence[Simulation], iters: int, max_candidates: int):
-    """One CCD + impulse step of every simulation in `sims`, packed into one pass of
-    launches per impulse iteration.  Returns the corrected end-of-step positions per
-    simulation (new tensors; the resident state is not modified)."""
-    toff = np.cumsum([0] + [s.num_triangles for s in sims]).tolist()
-    voff = np.cumsum([0] + [s.num_vertices for s in sims]).tolist()
-    if len(sims) == 1:
-        faces, t0, masses, x1 = sims[0].faces, sims[0].t0, sims[0].masses, sims[0].x1
-    else:
-        faces = torch.cat([s.faces + voff[i] for i, s in enumerate(sims)])
-        t0 = torch.cat([s.t0 for s in sims])
-        masses = torch.cat([s.masses for s in sims])
-        x1 = torch.cat([s.x1 for s in sims])
-    flat = faces.reshape(-1)
-    nv = x1.shape[0]
-    for _ in range(iters):
-        t1 = x1[faces].contiguous()
-        _, dx, cnt = ccc.compute_impulses_packed(t0, t1, masses, toff, max_candidates)
-        d = dx.reshape(-1, 3)
-        ok = torch.isfinite(d).all(dim=1)
-        d = torch.where(ok[:, None], d, torch.zeros_like(d))
-        k = torch.where(ok, cnt.reshape(-1).to(x1.dtype), torch.zeros_like(ok, dtype=x1.dtype))
-        s = torch.zeros((nv, 3), dtype=x1.dtype, device=x1.device).index_add_(0, flat, d)
-        c = torch.zeros((nv,), dtype=x1.dtype, device=x1.device).index_add_(0, flat, k)
-        x1 = x1 + torch.where(c[:, None] > 0, s / c.clamp(min=1)[:, None], torch.zeros_like(s))
-    return [x1[voff[i]:voff[i + 1]] for i in range(len(sims))]
+    def workspace_bytes(self) -> int:
+        return int(self.sim.workspace_bytes())
+
+    def step(self, workspace=None):
+        """One step on the current stream, advancing the state."""
+        self.sim.step(advance=True, workspace=workspace)
+        self.steps += 1
+
+    def positions(self):
+        return self.sim.positions()
 
 
 class CollisionServer:
-    """Worker slots over a Scheduler.  Each slot: one host thread, one HIP stream."""
+    """Worker slots over a Scheduler.  Each slot: one host thread, one HIP stream, one device
+    workspace sized for the largest simulation's step (grown if a simulation's candidate list
+    grows past it)."""
 
     def __init__(self, ccc, torch, sims: Sequence[Simulation], slots: int = 4, hbm_budget: Optional[int] = None,
-                 iters: int = 10, max_candidates: int = 512):
+                 keep_results: bool = False, autotune: bool = False):
         self.ccc, self.torch, self.sims = ccc, torch, list(sims)
-        self.iters, self.max_candidates = iters, max_candidates
-        fp = {i: footprint_bytes(s.num_triangles, max_candidates) for i, s in enumerate(self.sims)}
+        self.keep_results = keep_results
+        ws = max(s.workspace_bytes() for s in self.sims)
+        resident = sum(s.footprint() for s in self.sims)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
-            hbm_budget = int(0.6 * free)
-        self.slot_budget = max(hbm_budget // slots, 1)
-        self.scheduler = Scheduler(fp, self.slot_budget)
+            hbm_budget = int(0.9 * free) + resident
+        if resident + slots * ws > hbm_budget:
+            raise ValueError(f"the simulations ({resident} bytes) and {slots} slot workspaces ({ws} bytes each) do not fit "
+                             f"the HBM budget of {hbm_budget} bytes")
+        self.workspaces = [torch.empty(ws, dtype=torch.uint8, device="cuda:0") for _ in range(slots)]
+        self.streams = [torch.cuda.Stream() for _ in range(slots)]
+        if autotune:
+            # one timing per mesh-size bucket; the other simulations of a bucket reuse its choice
+            with torch.cuda.stream(self.streams[0]):
+                for s in self.sims:
+                    s.sim.autotune(workspace=self.workspaces[0])
+            self.streams[0].synchronize()
+        self.scheduler = Scheduler({i: s.workspace_bytes() for i, s in enumerate(self.sims)}, ws)
         self.clock0 = time.perf_counter()
         self.completed: List[Request] = []
         self.lock = threading.Lock()
-        self.streams = [torch.cuda.Stream() for _ in range(slots)]
         self.threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(slots)]
         self.errors: List[BaseException] = []
         for t in self.threads:
@@ -191,16 +192,20 @@ class CollisionServer:
                 r.batch_size = len(batch)
             try:
                 with torch.cuda.stream(stream):
-                    out = impulse_step(self.ccc, torch, [self.sims[r.sim] for r in batch], self.iters,
-                                       self.max_candidates)
+                    for r in batch:
+                        sim = self.sims[r.sim]
+                        need = sim.workspace_bytes()
+                        if need > self.workspaces[slot].numel():
+                            self.workspaces[slot] = torch.empty(need, dtype=torch.uint8, device="cuda:0")
+                        sim.step(workspace=self.workspaces[slot])
+                        if self.keep_results:
+                            r.result = sim.positions()[0]
+                        r.done = self.now()
                 stream.synchronize()
             except BaseException as e:       # reported by close()
                 self.errors.append(e)
-                out = [None] * len(batch)
-            t = self.now()
-            for r, o in zip(batch, out):
-                r.result = o
-                r.done = t
+                for r in batch:
+                    r.done = self.now() if r.done is None else r.done
             with self.lock:
                 self.completed.extend(batch)
             self.scheduler.release(batch)

@@ -65,51 +65,66 @@ def test_footprint_grows_with_mesh_and_capacity():
     assert serving.footprint_bytes(40000, 512) > a > serving.footprint_bytes(20000, 64)
 
 
+def _cloths(sizes, seed0=40):
+    from cccollisions_amd import scenes
+    return [scenes.folded_cloth(n, seed=seed0 + n) for n in sizes]
+
+
 @pytest.mark.gpu
 def test_results_same_alone_and_in_the_mix(ccc):
-    """Twelve requests over five folded cloths of different sizes, two worker slots: every
-    result equals the simulation's step computed alone (impulse sums reordered: 1e-9 of the
-    correction)."""
+    """Five resident cloths of different sizes, three requests each (fifteen, interleaved),
+    two worker slots: every request's resulting state is, bit for bit, the state the same
+    simulation reaches after the same number of steps run alone (deterministic sums)."""
     import torch
-    from cccollisions_amd import scenes
-    cl = [scenes.folded_cloth(n, seed=40 + n) for n in (6, 10, 14, 8, 12)]
-    sims = [serving.Simulation(torch, sc.faces, sc.x0, sc.x1, sc.corner_masses()) for sc in cl]
-    server = serving.CollisionServer(ccc, torch, sims, slots=2, iters=1, max_candidates=512)
-    reqs = [server.submit(i % len(sims)) for i in range(12)]
+    cl = _cloths((6, 10, 14, 8, 12))
+    mk = lambda sc: serving.Simulation(ccc, torch, sc.faces, sc.x0, sc.x1, sc.corner_masses(), iters=3)  # noqa: E731
+    sims = [mk(sc) for sc in cl]
+    server = serving.CollisionServer(ccc, torch, sims, slots=2, keep_results=True)
+    reqs = [server.submit(i % len(sims)) for i in range(3 * len(sims))]
     server.close()
+    assert all(s.steps == 3 for s in sims)
     assert max(r.batch_size for r in reqs) > 1
-    for r in reqs:
-        alone = serving.impulse_step(ccc, torch, [sims[r.sim]], 1, 512)[0]
-        corr = float((alone - sims[r.sim].x1).abs().max())
-        assert corr > 0
-        diff = float((r.result - alone).abs().max())
-        assert diff <= 1e-9 * corr + 1e-15, (r.sim, diff, corr)
-        assert r.done >= r.start >= r.arrival
+    for i, sc in enumerate(cl):
+        alone = mk(sc)
+        mine = [r for r in reqs if r.sim == i]
+        assert len(mine) == 3
+        for r in mine:
+            alone.step()
+            x0 = alone.positions()[0]
+            assert not torch.equal(x0, torch.from_numpy(sc.x0).to(x0.device))
+            assert torch.equal(r.result, x0), (i, float((r.result - x0).abs().max()))
+            assert r.done >= r.start >= r.arrival
 
 
 @pytest.mark.gpu
-def test_impulse_step_matches_the_oracle_step(ccc, oracle):
-    """The caller step on the GPU (bvh_motion + compute_impulses + mean-correction vertex
-    update, serving.impulse_step) against the same pass composed on the CPU from the oracle
-    and scenes.apply_corrections, for three passes in a row, each from the GPU's previous
-    positions: positions agree to 1e-9 of the pass's correction.  (Chained passes are not
-    compared end to end: the atomic summation order leaves 1e-17 differences in the
-    positions, and the next pass's borderline root and filter decisions amplify them.)"""
+def test_server_steps_match_the_oracle_step(ccc, oracle):
+    """Three advancing steps of one pass each: every step, from the GPU's state before it,
+    against the same pass composed on the CPU from the oracle and scenes.apply_corrections
+    (positions within 1e-9 of the pass's correction), and the advance rule x0 <- x1,
+    x1 <- x1 + (x1 - x0) exact."""
     import torch
     from cccollisions_amd import scenes
-    sc = scenes.folded_cloth(10, seed=7)
-    t0 = sc.x0[sc.faces]
+    sc = _cloths((10,), seed0=7)[0]
     m = sc.corner_masses()
-    x_prev = sc.x1.copy()
+    sim = serving.Simulation(ccc, torch, sc.faces, sc.x0, sc.x1, m, iters=1)
     for k in range(3):
-        sim = serving.Simulation(torch, sc.faces, sc.x0, x_prev, m)
-        got = serving.impulse_step(ccc, torch, [sim], 1, 512)[0].cpu().numpy()
-        bb, tree = oracle.bvh(t0, x_prev[sc.faces])[:2]
-        _, dx, cnt, _, _, counts = oracle.compute_impulses(bb, tree, t0, x_prev[sc.faces], m)
+        x0, x1 = (a.cpu().numpy() for a in sim.positions())
+        sim.step()
+        g0, g1 = (a.cpu().numpy() for a in sim.positions())
+        t0, t1 = x0[sc.faces], x1[sc.faces]
+        bb, tree = oracle.bvh(t0, t1)[:2]
+        _, dx, cnt, _, _, counts = oracle.compute_impulses(bb, tree, t0, t1, m)
         assert counts["collisions"] > 0
-        ref = scenes.apply_corrections(sc.faces, x_prev, dx, cnt)
-        corr = float(np.abs(ref - x_prev).max())
+        ref = scenes.apply_corrections(sc.faces, x1, dx, cnt)
+        corr = float(np.abs(ref - x1).max())
         assert corr > 0
-        diff = float(np.abs(got - ref).max())
-        assert diff <= 1e-9 * corr + 1e-15, (k, diff, corr)
-        x_prev = got
+        assert float(np.abs(g0 - ref).max()) <= 1e-9 * corr + 1e-15, k
+        np.testing.assert_array_equal(g1, g0 + (g0 - x0))
+
+
+@pytest.mark.gpu
+def test_server_rejects_an_hbm_budget_it_cannot_hold(ccc):
+    import torch
+    sims = [serving.Simulation(ccc, torch, sc.faces, sc.x0, sc.x1, sc.corner_masses(), iters=1) for sc in _cloths((6, 8))]
+    with pytest.raises(ValueError, match="HBM budget"):
+        serving.CollisionServer(ccc, torch, sims, slots=2, hbm_budget=1 << 20)
