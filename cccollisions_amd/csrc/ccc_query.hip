@@ -1120,6 +1120,11 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     constexpr int kStageCap = MODE == MODE_IMPULSE ? kImpStage : kCollectFlush + 3 * NT;
     __shared__ int64_t s_item[kStageCap];
     __shared__ T s_root[kStageCap];
+    // roots waiting for a whole round of the filter phase: fewer than NT, carried from tile to
+    // tile as (item, root) so that the filter runs full waves
+    __shared__ int64_t s_citem[NT];
+    __shared__ T s_croot[NT];
+    __shared__ int s_ncarry;
     __shared__ unsigned long long s_obase;
     __shared__ int s_fill;
     const int b = blockIdx.y, tid = threadIdx.x;
@@ -1134,7 +1139,10 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
     long long *det = DET ? det_all + (int64_t)b * n * 3 * kDetWords : nullptr;
     unsigned long long nroots = 0, ncoll = 0;
-    if (tid == 0) s_fill = 0;
+    if (tid == 0) {
+        s_fill = 0;
+        s_ncarry = 0;
+    }
     __syncthreads();
 
     // Staged collisions are scattered kFlush at a time: their corner impulses are first
@@ -1220,6 +1228,19 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
         __syncthreads();
     };
 
+    // Filter of one root (filterCollisions); accepted collisions join the stage.
+    auto filter_one = [&](int64_t rit, T root) {
+        int64_t x, y;
+        int type;
+        unpack_item(rit, x, y, type);
+        const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
+        if (filter_collision(m, type, root)) {
+            const int p = atomicAdd(&s_fill, 1);
+            s_item[p] = rit;
+            s_root[p] = root;
+        }
+    };
+
     // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
     // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
     const int64_t ntiles = (M + NT - 1) / NT;
@@ -1280,27 +1301,33 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
             }
         }
         __syncthreads();
-        // ---- C: filter one root record per thread
-        const int nr = s_nroot;
-        for (int base = 0; base < nr; base += NT) {
+        // ---- C: filter one root per thread in whole rounds over the carried roots followed
+        // by this tile's; the rest (fewer than NT) is carried to the next tile
+        const int nr = s_nroot, nc = s_ncarry, total = nc + nr;
+        const int full = total / NT * NT;
+        for (int base = 0; base < full; base += NT) {
             const int j = base + tid;
-            bool ok = false;
-            int64_t rit = 0;
-            T root = 0;
-            if (j < nr) {
-                rit = s_rec[s_rrec[j]].item;
-                root = s_rroot[j];
-                int64_t x, y;
-                int type;
-                unpack_item(rit, x, y, type);
-                const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
-                ok = filter_collision(m, type, root);
+            int64_t rit;
+            T root;
+            if (j < nc) {
+                rit = s_citem[j];
+                root = s_croot[j];
+            } else {
+                rit = s_rec[s_rrec[j - nc]].item;
+                root = s_rroot[j - nc];
             }
-            if (ok) {
-                const int p = atomicAdd(&s_fill, 1);
-                s_item[p] = rit;
-                s_root[p] = root;
+            filter_one(rit, root);
+        }
+        __syncthreads();      // the carried roots have been read
+        {
+            // with a whole round the carry was consumed and the rest is this tile's tail;
+            // without one, this tile's roots join the carry
+            const int from = full > 0 ? full - nc : 0, to = full > 0 ? 0 : nc;
+            for (int k = tid; k < total - full - to; k += NT) {
+                s_citem[to + k] = s_rec[s_rrec[from + k]].item;
+                s_croot[to + k] = s_rroot[from + k];
             }
+            if (tid == 0) s_ncarry = total - full;
         }
         // ---- D: scatter the staged impulses in whole chunks; the rest moves to the front
         if (MODE == MODE_IMPULSE) {
@@ -1329,6 +1356,10 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
         }
         __syncthreads();
     }
+    // the carried roots' last, partial round
+    __syncthreads();
+    if (tid < s_ncarry) filter_one(s_citem[tid], s_croot[tid]);
+    __syncthreads();
     if (MODE == MODE_IMPULSE) {
         const int f = s_fill;
         flush(f);
