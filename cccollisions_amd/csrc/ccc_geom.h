@@ -301,23 +301,10 @@ __device__ __forceinline__ double dd_cube_lead(double b) {
     return q + t;
 }
 
-// collisions_continuous.cu:448-464 add_root (window [-1e-16, 1+1e-16], slots x,y,z).  The
-// reference's parameter is `T root`: the double root is rounded to T at the call and the
-// window is tested on the rounded value (for float, a root in (1, 1 + 2^-24] becomes 1.0f and
-// is kept).
-template <typename T> __device__ __forceinline__ void add_root(T *slots, double root, int &n) {
-    const T r = (T)root;
-    if ((double)r < -1e-16 || (double)r > 1 + 1e-16) return;
-    if (n == 0) slots[0] = r;
-    else if (n == 1) slots[1] = r;
-    else if (n == 2) slots[2] = r;
-    n++;
-}
-
 // collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes, with the
 // portable cbrt / acos / sin / cos of ccc_libm.h.  Split in two so that a kernel can bin
 // items by branch between the halves: cubic_prepare classifies and evaluates f, g, h;
-// cubic_finish produces the roots.  solve_cubic = prepare + finish.
+// cubic_roots produces the roots.
 enum : int { CUBIC_NONE = 0, CUBIC_QUADRATIC = 1, CUBIC_TRIPLE = 2, CUBIC_TRIG = 3, CUBIC_CARDANO = 4 };
 
 struct CubicState { double a, b, c, d, f, g, h; };
@@ -341,58 +328,59 @@ __device__ __forceinline__ int cubic_prepare(double a, double b, double c, doubl
     return h <= 0 ? CUBIC_TRIG : CUBIC_CARDANO;
 }
 
-template <typename T> __device__ __forceinline__ void cubic_finish(int code, const CubicState &st, T *slots) {
+// The roots of a classified cubic (collisions_continuous.cu:537-611), passed one at a time to
+// emit((T)root) when they survive both of the reference's windows: add_root keeps roots in
+// [-1e-16, 1 + 1e-16] (:448-464) and flatten_cubic_roots then keeps those in [0, 1]
+// (:743-763), so a root is kept exactly when its value rounded to T lies in [0, 1] (NaN
+// fails both); equal roots are emitted twice.  The trigonometric and Cardano forms share
+// their first square root and cube root: the trigonometric form takes sqrt(g^2/4 - h) and
+// its cube root, Cardano sqrt(h) and the cube root of |R| (cbrt(R) for R >= 0 and
+// -cbrt(-R) otherwise are +-cbrt(|R|)), so a wave holding both forms evaluates them once.
+template <typename T, typename Emit>
+__device__ __forceinline__ void cubic_roots(int code, const CubicState &st, Emit emit) {
     const double a = st.a, b = st.b, c = st.c, d = st.d, g = st.g, h = st.h;
-    int n = 0;
-    slots[0] = slots[1] = slots[2] = (T)-1;
+    auto keep = [&](double x) {
+        const T r = (T)x;
+        if (r >= (T)0 && r <= (T)1) emit(r);
+    };
     if (code == CUBIC_QUADRATIC) {
         // :537-547
         double D = c * c - 4.0 * b * d;
         if (D >= 0) {
             D = sqrt(D);
-            double x1 = (-c + D) / (2.0 * b);
-            double x2 = (-c - D) / (2.0 * b);
-            add_root(slots, x1, n);
-            add_root(slots, x2, n);
+            keep((-c + D) / (2.0 * b));
+            keep((-c - D) / (2.0 * b));
         }
     } else if (code == CUBIC_TRIPLE) {
         // :564-575 triple root through a float k
         float k = (float)(d / a);
-        double x1;
-        if (k >= 0) x1 = (double)(pm_cbrtf(k) * -1);
-        else x1 = (double)pm_cbrtf(-k);
-        add_root(slots, x1, n);
-    } else if (code == CUBIC_TRIG) {
-        // :576-591 three real roots (trigonometric form)
-        double i = sqrt(((g * g) / 4.0) - h);
-        double j = pm_cbrt(i);
-        double k = pm_acos(-(g / (2 * i)));
-        double L = j * -1;
-        // cos(k / 3) appears twice in the formulas; evaluated once (the same value)
-        double M = pm_cos(k / 3.0);
-        double N = sqrt(3.0) * pm_sin(k / 3.0);
-        double P = (b / (3.0 * a)) * -1;
-        double x1 = 2 * j * M - (b / (3.0 * a));
-        double x2 = L * (M + N) + P;
-        double x3 = L * (M - N) + P;
-        add_root(slots, x1, n);
-        add_root(slots, x2, n);
-        add_root(slots, x3, n);
-    } else if (code == CUBIC_CARDANO) {
-        // :592-611 one real root (Cardano)
-        double R = -(g / 2.0) + sqrt(h);
-        double S = R >= 0 ? pm_cbrt(R) : -pm_cbrt(-R);
-        double V = -(g / 2.0) - sqrt(h);
-        double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
-        double x1 = (S + U) - (b / (3.0 * a));
-        add_root(slots, x1, n);
+        if (k >= 0) keep((double)(pm_cbrtf(k) * -1));
+        else keep((double)pm_cbrtf(-k));
+    } else if (code == CUBIC_TRIG || code == CUBIC_CARDANO) {
+        const bool trig = code == CUBIC_TRIG;
+        const double sq = sqrt(trig ? ((g * g) / 4.0) - h : h);
+        const double R = -(g / 2.0) + sq;
+        const double cb = pm_cbrt(trig ? sq : fabs(R));
+        const double b3a = b / (3.0 * a);
+        if (trig) {
+            // :576-591 three real roots: i = sq, j = cb
+            const double k = pm_acos(-(g / (2 * sq)));
+            const double L = cb * -1;
+            // cos(k / 3) appears twice in the formulas; evaluated once (the same value)
+            const double M = pm_cos(k / 3.0);
+            const double N = sqrt(3.0) * pm_sin(k / 3.0);
+            const double P = b3a * -1;
+            keep(2 * cb * M - b3a);
+            keep(L * (M + N) + P);
+            keep(L * (M - N) + P);
+        } else {
+            // :592-611 one real root
+            const double S = R >= 0 ? cb : -cb;
+            const double V = -(g / 2.0) - sq;
+            const double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
+            keep((S + U) - b3a);
+        }
     }
-}
-
-template <typename T> __device__ __forceinline__ void solve_cubic(const T *coef, T *slots) {
-    CubicState st;
-    const int code = cubic_prepare(coef[0], coef[1], coef[2], coef[3], st);
-    cubic_finish(code, st, slots);
 }
 
 // Positions at time t: figures.cu:112-117.

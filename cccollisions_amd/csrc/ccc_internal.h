@@ -220,6 +220,16 @@ __device__ __forceinline__ unsigned long long readfirstlane_u64(unsigned long lo
     unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
+// Slot of the calling lane in a list whose length is the LDS counter *counter: every active
+// lane appends one entry, the wave's entries go in lane order after one atomic.
+__device__ __forceinline__ int wave_append1(int *counter) {
+    const unsigned long long act = __ballot(1);
+    const unsigned below = lanes_below(act);
+    int base = 0;
+    if (below == 0) base = atomicAdd(counter, (int)__popcll(act));
+    base = __builtin_amdgcn_readfirstlane(base);
+    return base + (int)below;
+}
 // Reserve `cnt` consecutive slots (cnt < 16) per active lane of the wave from `counter`
 // with a single atomic; returns this lane's first slot.
 __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *counter, unsigned cnt) {

@@ -144,18 +144,21 @@ __device__ __forceinline__ double pm_asin_small(double z) {
     return fma(z * t, estrin<AsinCoef, 13>(t), z);
 }
 
+// |x| <= 1/2: pi/2 - asin(x); otherwise 2 asin(sqrt((1 - |x|) / 2)), reflected about pi/2
+// for x < 0.  The one asin_small evaluation takes the argument of the lane's case, so that
+// a wave holding both cases runs the polynomial once.
 __device__ __forceinline__ double pm_acos(double x) {
     if (x != x) return x;
     if (x > 1.0 || x < -1.0) return NAN;
     const double ax = fabs(x);
-    if (ax <= 0.5) {
-        const double a = pm_asin_small(x);
+    const bool small = ax <= 0.5;
+    const double a = pm_asin_small(small ? x : sqrt((1.0 - ax) * 0.5));
+    if (small) {
         const double r = CCC_PIO2_HI - a;
         const double err = (CCC_PIO2_HI - r) - a;
         return r + (err + CCC_PIO2_LO);
     }
-    const double z = sqrt((1.0 - ax) * 0.5);
-    const double s = 2.0 * pm_asin_small(z);
+    const double s = 2.0 * a;
     if (x > 0) return s;
     const double r = CCC_PI_HI - s;
     const double err = (CCC_PI_HI - r) - s;

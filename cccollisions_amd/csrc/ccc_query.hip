@@ -1138,6 +1138,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
     int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
     long long *det = DET ? det_all + (int64_t)b * n * 3 * kDetWords : nullptr;
+    // block-wide counts, the same in every lane (scalar registers); lane 0 adds them at the end
     unsigned long long nroots = 0, ncoll = 0;
     if (tid == 0) {
         s_fill = 0;
@@ -1235,7 +1236,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
         unpack_item(rit, x, y, type);
         const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
         if (filter_collision(m, type, root)) {
-            const int p = atomicAdd(&s_fill, 1);
+            const int p = wave_append1(&s_fill);
             s_item[p] = rit;
             s_root[p] = root;
         }
@@ -1270,8 +1271,11 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
         if (tid < 5) s_cnt[tid] = 0;
         if (tid == 0) s_nroot = 0;
         __syncthreads();
+        // binned by branch: one LDS atomic per wave and branch
         int pos = 0;
-        if (code != CUBIC_NONE) pos = atomicAdd(&s_cnt[code], 1);
+#pragma unroll
+        for (int cc = CUBIC_QUADRATIC; cc <= CUBIC_CARDANO; ++cc)
+            if (code == cc) pos = wave_append1(&s_cnt[cc]);
         __syncthreads();
         const int o2 = s_cnt[CUBIC_QUADRATIC], o3 = o2 + s_cnt[CUBIC_TRIPLE], o4 = o3 + s_cnt[CUBIC_TRIG];
         const int tot = o4 + s_cnt[CUBIC_CARDANO];
@@ -1288,19 +1292,14 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
         if (tid < tot) {
             const int c = tid < o2 ? CUBIC_QUADRATIC : tid < o3 ? CUBIC_TRIPLE : tid < o4 ? CUBIC_TRIG : CUBIC_CARDANO;
             const CubicState rs = rec_load(c, s_rec[tid]);
-            T sl[3];
-            cubic_finish(c, rs, sl);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (sl[k] >= 0 && sl[k] <= 1) {
-                    ++nroots;
-                    const int p = atomicAdd(&s_nroot, 1);
-                    s_rrec[p] = (unsigned short)tid;
-                    s_rroot[p] = sl[k];
-                }
-            }
+            cubic_roots<T>(c, rs, [&](T r) {
+                const int p = wave_append1(&s_nroot);
+                s_rrec[p] = (unsigned short)tid;
+                s_rroot[p] = r;
+            });
         }
         __syncthreads();
+        nroots += (unsigned long long)__builtin_amdgcn_readfirstlane(s_nroot);
         // ---- C: filter one root per thread in whole rounds over the carried roots followed
         // by this tile's; the rest (fewer than NT) is carried to the next tile
         const int nr = s_nroot, nc = s_ncarry, total = nc + nr;
@@ -1336,7 +1335,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
             if (f >= kFlush) {
                 const int done = f - f % kFlush;
                 flush(done);
-                if (tid == 0) ncoll += (unsigned long long)done;
+                ncoll += (unsigned long long)done;
                 // [done, f) and [0, f - done) do not overlap: f - done < kFlush <= done
                 if (tid < f - done) {
                     s_item[tid] = s_item[done + tid];
@@ -1363,12 +1362,12 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     if (MODE == MODE_IMPULSE) {
         const int f = s_fill;
         flush(f);
-        if (tid == 0) ncoll += (unsigned long long)f;
+        ncoll += (unsigned long long)f;
     } else {
         write_rows(s_fill);
     }
-    block_add(&ctr->roots, nroots);
-    if (MODE == MODE_IMPULSE) block_add(&ctr->coll, ncoll);
+    block_add(&ctr->roots, tid == 0 ? nroots : 0ull);
+    if (MODE == MODE_IMPULSE) block_add(&ctr->coll, tid == 0 ? ncoll : 0ull);
 }
 
 // compute_impulses_from_collisions (collisions_impulses.cu:1736-1880) over given rows;
