@@ -445,10 +445,12 @@ template <typename T> __device__ __forceinline__ bool filter_collision(const MTe
         T s = ns / den;
         T q = nq / den;
         return !(s < 0 || s > 1 || q < 0 || q > 1);
-    } else if (type < 12) {
-        return is_inside(p[0], p[1], p[2], p[3]);
     }
-    return is_inside(p[1], p[2], p[3], p[0]);
+    // vertex-face: types 9-11 test vertex p[3] against face p[0..2], types 12-14 vertex p[0]
+    // against face p[1..3]; one is_inside on the lane's corners (a wave holding both runs it
+    // once)
+    const bool inv = type >= 12;
+    return is_inside(inv ? p[1] : p[0], inv ? p[2] : p[1], inv ? p[3] : p[2], inv ? p[0] : p[3]);
 }
 
 // Impulse of one accepted collision: collisions_impulses.cu:24-123 (mass quad),
