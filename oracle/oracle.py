@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/_build/libccc_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of bench.py, as the checker.  The product package
-(``cccollisions_amd``) never imports it.
+``cpu_baseline`` leg of bench.py, to check and time the GPU path against.  The product
+package (``cccollisions_amd``) never imports it.
 
 Every function takes/returns numpy arrays in the reference's layouts:
 triangle soups ``[N, 3, 3]``, BVH ``bboxes [2N-1, 6]`` / ``tree [2N-1, 4]`` int64,
