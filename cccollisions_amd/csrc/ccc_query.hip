@@ -664,7 +664,10 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 //   SUB_STATIC: boxes grown by the threshold (proximity_static.cu:11-118) -> packed items;
 //   SUB_MINCOL: swept boxes, written as (x, y, type) rows (find_minimal_candidates, :321-338).
 // Items stay in pair order: pair u * kThreads + t of a tile comes after every pair
-// u' * kThreads + t' < u * kThreads + t.  The moving masks are computed one axis at a time
+// u' * kThreads + t' < u * kThreads + t.  SUB_MOVING keeps two such runs per chunk, the
+// edge-edge items from the front of the list (counted in `items`) and the vertex-face items
+// from its back (item j at capM - 1 - j), both counted in `items_split` (edge-edge in the low
+// and vertex-face in the high 32 bits), so that the narrow phase's tiles hold one kind.  The moving masks are computed one axis at a time
 // from the vertex records (moving_mask_axes), which keeps the registers low enough for PP
 // pairs per thread.  A tile's items are assembled in LDS (a 2-byte entry per item: pair in
 // tile and type) and written as one contiguous run.
@@ -691,15 +694,54 @@ __device__ __forceinline__ unsigned block_scan_excl(unsigned cnt, unsigned &tota
     return before + below;
 }
 
+// Two exclusive block-wide scans in one pass: counts a < 16 and b < 8 per thread; returns the
+// thread's offsets packed as (a offset) | (b offset) << 16 and the block's sums packed the
+// same way in `total` (a block sum stays below 2^16: at most 15 * NT).  Every thread calls it.
+template <int NT = kThreads>
+__device__ __forceinline__ unsigned block_scan_excl2(unsigned a, unsigned b, unsigned &total) {
+    static_assert(15 * NT < 65536, "packed halves do not carry");
+    __shared__ unsigned s_part[NT / 64 + 1];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned below = 0, mine = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        const unsigned long long m = __ballot((a >> bit) & 1u);
+        below += lanes_below(m) << bit;
+        mine += (unsigned)__popcll(m) << bit;
+    }
+#pragma unroll
+    for (int bit = 0; bit < 3; ++bit) {
+        const unsigned long long m = __ballot((b >> bit) & 1u);
+        below += lanes_below(m) << (16 + bit);
+        mine += (unsigned)__popcll(m) << (16 + bit);
+    }
+    if (lane == 0) s_part[w] = mine;
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) {
+        const unsigned v = s_part[q];
+        if (q < w) before += v;
+        tot += v;
+    }
+    __syncthreads();
+    total = tot;
+    return before + below;
+}
+
 template <typename T, int KIND, int PP>
 __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
                                                       const T *__restrict__ src_all, int64_t n, T thr,
                                                       int64_t *__restrict__ out_all, int64_t capM,
                                                       DevCounters *__restrict__ ctr_all) {
     static_assert(PP >= 1 && PP * kThreads <= 4096, "pair index and type fit 16 bits");
-    __shared__ unsigned short s_ent[15 * PP * kThreads];    // (pair in tile) << 4 | type
+    // (pair in tile) << 4 | type; SUB_MOVING: edge-edge entries from the front, vertex-face
+    // entries from the back (at most 9 and 6 per pair: the two runs never meet)
+    constexpr int kEnt = 15 * PP * kThreads;
+    constexpr bool SPLIT = KIND == SUB_MOVING;
+    __shared__ unsigned short s_ent[kEnt];
     __shared__ int2 s_pair[PP * kThreads];
-    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_base, s_base_vf;
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
     const unsigned long long kc = ctr->cand;
@@ -712,7 +754,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     constexpr int kTile = PP * kThreads;
     const int64_t ntiles = K > c0 ? (K - c0 + kTile - 1) / kTile : 0;
     CCC_FOR_TILES(tile, ntiles) {
-        unsigned tot = 0;
+        unsigned tot = 0, tot_vf = 0;
 #pragma unroll
         for (int u = 0; u < PP; ++u) {
             const int64_t i = c0 + tile * kTile + u * kThreads + tid;
@@ -732,16 +774,45 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
                 }
             }
             s_pair[u * kThreads + tid] = make_int2(x, y);
-            unsigned t;
-            unsigned k = tot + block_scan_excl((unsigned)__popc(mask), t);
-            tot += t;
-            while (mask) {
-                const int type = __ffs(mask) - 1;
-                mask &= mask - 1;
-                s_ent[k++] = (unsigned short)(((u * kThreads + tid) << 4) | type);
+            if (SPLIT) {
+                uint32_t mee = mask & 0x1ffu, mvf = mask >> 9;
+                unsigned t;
+                const unsigned k2 = block_scan_excl2((unsigned)__popc(mee), (unsigned)__popc(mvf), t);
+                unsigned ke = tot + (k2 & 0xffffu), kv = tot_vf + (k2 >> 16);
+                tot += t & 0xffffu;
+                tot_vf += t >> 16;
+                while (mee) {
+                    const int type = __ffs(mee) - 1;
+                    mee &= mee - 1;
+                    s_ent[ke++] = (unsigned short)(((u * kThreads + tid) << 4) | type);
+                }
+                while (mvf) {
+                    const int type = 9 + __ffs(mvf) - 1;
+                    mvf &= mvf - 1;
+                    s_ent[kEnt - 1 - kv++] = (unsigned short)(((u * kThreads + tid) << 4) | type);
+                }
+            } else {
+                unsigned t;
+                unsigned k = tot + block_scan_excl((unsigned)__popc(mask), t);
+                tot += t;
+                while (mask) {
+                    const int type = __ffs(mask) - 1;
+                    mask &= mask - 1;
+                    s_ent[k++] = (unsigned short)(((u * kThreads + tid) << 4) | type);
+                }
             }
         }
-        if (tid == 0) s_base = tot ? atomicAdd(&ctr->items, (unsigned long long)tot) : 0ull;
+        if (tid == 0) {
+            if (SPLIT) {
+                // one reservation for both runs (the counter is one contended address)
+                const unsigned long long add = (unsigned long long)tot | ((unsigned long long)tot_vf << 32);
+                const unsigned long long old = add ? atomicAdd(&ctr->items_split, add) : 0ull;
+                s_base = old & 0xffffffffull;
+                s_base_vf = old >> 32;
+            } else {
+                s_base = tot ? atomicAdd(&ctr->items, (unsigned long long)tot) : 0ull;
+            }
+        }
         __syncthreads();
         const unsigned long long base = s_base;
         const unsigned long long room = base < (unsigned long long)capM ? (unsigned long long)capM - base : 0ull;
@@ -754,6 +825,17 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
                 out[3 * slot] = pr.x; out[3 * slot + 1] = pr.y; out[3 * slot + 2] = (int64_t)(e & 15u);
             } else {
                 out[slot] = pack_item(pr.x, pr.y, (int)(e & 15u));
+            }
+        }
+        if (SPLIT) {
+            // vertex-face item j of the chunk sits at capM - 1 - j
+            const unsigned long long bv = s_base_vf;
+            const unsigned long long rv = bv < (unsigned long long)capM ? (unsigned long long)capM - bv : 0ull;
+            const unsigned limv = rv < tot_vf ? (unsigned)rv : tot_vf;
+            for (unsigned k = tid; k < limv; k += kThreads) {
+                const unsigned e = s_ent[kEnt - 1 - k];
+                const int2 pr = s_pair[e >> 4];
+                out[capM - 1 - (int64_t)(bv + k)] = pack_item(pr.x, pr.y, (int)(e & 15u));
             }
         }
         __syncthreads();
@@ -1129,8 +1211,10 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     __shared__ int s_fill;
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
-    const unsigned long long mc = ctr->items;
-    const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
+    // edge-edge items [0, ME) from the front of the list, vertex-face items from the back
+    const unsigned long long ms = ctr->items_split, mc = ms & 0xffffffffull, mv = ms >> 32;
+    const int64_t ME = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
+    const int64_t MV = (int64_t)(mv < (unsigned long long)(capM - ME) ? mv : (unsigned long long)(capM - ME));
     const int64_t *items = items_all + (int64_t)b * capM;
     const T *tv = tv_all + (int64_t)b * n * 18;
     const T *masses = MODE == MODE_IMPULSE ? masses_all + (int64_t)b * n * 3 : nullptr;
@@ -1244,12 +1328,21 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
     // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
-    const int64_t ntiles = (M + NT - 1) / NT;
+    // Tiles [0, tilesE) hold edge-edge items, the rest vertex-face items, so a tile's roots
+    // and collisions take one branch of the filter and of the impulse in the whole wave.
+    const int64_t tilesE = (ME + NT - 1) / NT;
+    const int64_t ntiles = tilesE + (MV + NT - 1) / NT;
     const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
     const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
     auto item_of = [&](int64_t k) -> int64_t {
-        const int64_t i = (tbase + k) * NT + tid;
-        return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
+        const int64_t t = tbase + k;
+        if (k >= tper || t >= ntiles) return -1;
+        if (t < tilesE) {
+            const int64_t i = t * NT + tid;
+            return i < ME ? items[i] : (int64_t)-1;
+        }
+        const int64_t j = (t - tilesE) * NT + tid;
+        return j < MV ? items[capM - 1 - j] : (int64_t)-1;
     };
     int64_t next_item = item_of(blockIdx.x / kXcds);
     for (int64_t tk = blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
@@ -1612,8 +1705,10 @@ struct Query {
 // End of a pair chunk: its sub-primitive items are done; the next chunk refills the list.
 __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
     for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
-        ctr[b].items_done += ctr[b].items;
+        const unsigned long long sp = ctr[b].items_split;
+        ctr[b].items_done += ctr[b].items + (sp & 0xffffffffull) + (sp >> 32);
         ctr[b].items = 0;
+        ctr[b].items_split = 0;
     }
 }
 
@@ -1621,6 +1716,13 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 // another size): the item list holds at most 15 items per pair of a chunk, 4 GiB at this
 // size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
+// Largest chunk: the moving narrow phase counts a chunk's edge-edge items (at most 9 per
+// pair) in 32 bits.
+constexpr int64_t kMaxPairChunk = (int64_t)1 << 28;
+static int64_t pair_chunk_of(const ccc_context *ctx, int64_t chunk_ext, int64_t pairs, int64_t B) {
+    const int64_t budget = chunk_ext > 0 ? chunk_ext * B : ctx->pair_chunk > 0 ? ctx->pair_chunk : kPairChunk;
+    return std::max<int64_t>(std::min<int64_t>(std::min<int64_t>(pairs, budget / B), kMaxPairChunk), 1);
+}
 constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim
 
 // Grid of the tile-looping kernels: one block per tile up to 128 blocks per CU, each block
@@ -1808,8 +1910,7 @@ static ccc_status run_query(const Query &q) {
         // sub-primitive items and the narrow phase run over chunks of the pair list; the
         // chunk bounds the pairs of all batch elements together (the item buffer holds B
         // chunks), so the workspace does not grow with the batch
-        const int64_t budget = q.chunk_ext > 0 ? q.chunk_ext * B : q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
-        int64_t chunk = std::max<int64_t>(std::min<int64_t>(kmax, budget / B), 1);
+        int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, kmax, B);
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
@@ -1916,8 +2017,7 @@ static size_t resident_bytes(const Query &q) {
     tot += r(sizeof(float) * B * (int64_t)(gT / kFrontier) * 6);
     const int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
     if (!q.pairs_ext) tot += r(sizeof(int2) * B * capK);
-    const int64_t budget = q.chunk_ext > 0 ? q.chunk_ext * B : q.ctx->pair_chunk > 0 ? q.ctx->pair_chunk : kPairChunk;
-    const int64_t chunk = std::max<int64_t>(std::min<int64_t>(capK, budget / B), 1);
+    const int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, capK, B);
     tot += r(esz * B * n * 18) + r(sizeof(int64_t) * B * 15 * chunk);
     return tot;
 }
