@@ -304,7 +304,7 @@ __device__ __forceinline__ double dd_cube_lead(double b) {
 // collisions_continuous.cu:511-613 solveCubic, in double precision for both dtypes, with the
 // portable cbrt / acos / sin / cos of ccc_libm.h.  Split in two so that a kernel can bin
 // items by branch between the halves: cubic_prepare classifies and evaluates f, g, h;
-// cubic_roots produces the roots.
+// cubic_candidates and cubic_roots produce the roots.
 enum : int { CUBIC_NONE = 0, CUBIC_QUADRATIC = 1, CUBIC_TRIPLE = 2, CUBIC_TRIG = 3, CUBIC_CARDANO = 4 };
 
 struct CubicState { double a, b, c, d, f, g, h; };
@@ -328,34 +328,28 @@ __device__ __forceinline__ int cubic_prepare(double a, double b, double c, doubl
     return h <= 0 ? CUBIC_TRIG : CUBIC_CARDANO;
 }
 
-// The roots of a classified cubic (collisions_continuous.cu:537-611), passed one at a time to
-// emit((T)root) when they survive both of the reference's windows: add_root keeps roots in
-// [-1e-16, 1 + 1e-16] (:448-464) and flatten_cubic_roots then keeps those in [0, 1]
-// (:743-763), so a root is kept exactly when its value rounded to T lies in [0, 1] (NaN
-// fails both); equal roots are emitted twice.  The trigonometric and Cardano forms share
-// their first square root and cube root: the trigonometric form takes sqrt(g^2/4 - h) and
-// its cube root, Cardano sqrt(h) and the cube root of |R| (cbrt(R) for R >= 0 and
-// -cbrt(-R) otherwise are +-cbrt(|R|)), so a wave holding both forms evaluates them once.
-template <typename T, typename Emit>
-__device__ __forceinline__ void cubic_roots(int code, const CubicState &st, Emit emit) {
+// The candidate roots of a classified cubic (collisions_continuous.cu:537-611) in the
+// reference's order, in x0, x1, x2; returns their number (0 to 3).  The trigonometric and
+// Cardano forms share their first square root and cube root: the trigonometric form takes
+// sqrt(g^2/4 - h) and its cube root, Cardano sqrt(h) and the cube root of |R| (cbrt(R) for
+// R >= 0 and -cbrt(-R) otherwise are +-cbrt(|R|)), so a wave holding both forms evaluates
+// them once.
+__device__ __forceinline__ int cubic_candidates(int code, const CubicState &st, double &x0, double &x1, double &x2) {
     const double a = st.a, b = st.b, c = st.c, d = st.d, g = st.g, h = st.h;
-    auto keep = [&](double x) {
-        const T r = (T)x;
-        if (r >= (T)0 && r <= (T)1) emit(r);
-    };
     if (code == CUBIC_QUADRATIC) {
         // :537-547
         double D = c * c - 4.0 * b * d;
         if (D >= 0) {
             D = sqrt(D);
-            keep((-c + D) / (2.0 * b));
-            keep((-c - D) / (2.0 * b));
+            x0 = (-c + D) / (2.0 * b);
+            x1 = (-c - D) / (2.0 * b);
+            return 2;
         }
     } else if (code == CUBIC_TRIPLE) {
         // :564-575 triple root through a float k
         float k = (float)(d / a);
-        if (k >= 0) keep((double)(pm_cbrtf(k) * -1));
-        else keep((double)pm_cbrtf(-k));
+        x0 = k >= 0 ? (double)(pm_cbrtf(k) * -1) : (double)pm_cbrtf(-k);
+        return 1;
     } else if (code == CUBIC_TRIG || code == CUBIC_CARDANO) {
         const bool trig = code == CUBIC_TRIG;
         const double sq = sqrt(trig ? ((g * g) / 4.0) - h : h);
@@ -370,17 +364,38 @@ __device__ __forceinline__ void cubic_roots(int code, const CubicState &st, Emit
             const double M = pm_cos(k / 3.0);
             const double N = sqrt(3.0) * pm_sin(k / 3.0);
             const double P = b3a * -1;
-            keep(2 * cb * M - b3a);
-            keep(L * (M + N) + P);
-            keep(L * (M - N) + P);
-        } else {
-            // :592-611 one real root
-            const double S = R >= 0 ? cb : -cb;
-            const double V = -(g / 2.0) - sq;
-            const double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
-            keep((S + U) - b3a);
+            x0 = 2 * cb * M - b3a;
+            x1 = L * (M + N) + P;
+            x2 = L * (M - N) + P;
+            return 3;
         }
+        // :592-611 one real root
+        const double S = R >= 0 ? cb : -cb;
+        const double V = -(g / 2.0) - sq;
+        const double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
+        x0 = (S + U) - b3a;
+        return 1;
     }
+    return 0;
+}
+
+// The roots of a classified cubic that survive both of the reference's windows, in order,
+// in r0, r1, r2; returns their number.  add_root keeps roots in [-1e-16, 1 + 1e-16]
+// (:448-464) and flatten_cubic_roots then keeps those in [0, 1] (:743-763), so a root is
+// kept exactly when its value rounded to T lies in [0, 1] (NaN fails both); equal roots are
+// kept twice.
+template <typename T>
+__device__ __forceinline__ int cubic_roots(int code, const CubicState &st, T &r0, T &r1, T &r2) {
+    double x0 = 0, x1 = 0, x2 = 0;
+    const int nc = code == CUBIC_NONE ? 0 : cubic_candidates(code, st, x0, x1, x2);
+    const T c0 = (T)x0, c1 = (T)x1, c2 = (T)x2;
+    const bool k0 = nc > 0 && c0 >= (T)0 && c0 <= (T)1;
+    const bool k1 = nc > 1 && c1 >= (T)0 && c1 <= (T)1;
+    const bool k2 = nc > 2 && c2 >= (T)0 && c2 <= (T)1;
+    r0 = k0 ? c0 : k1 ? c1 : c2;
+    r1 = k0 && k1 ? c1 : c2;
+    r2 = c2;
+    return (int)k0 + (int)k1 + (int)k2;
 }
 
 // Positions at time t: figures.cu:112-117.

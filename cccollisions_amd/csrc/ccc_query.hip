@@ -873,7 +873,7 @@ enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
 constexpr int kCcdThreads = 64;
 // The collect mode keeps two-wave tiles: its output stage needs fewer registers than the
 // scatter, and one-wave tiles measured slower there (20.0 against 18.3 ms per pass).
-template <int MODE> constexpr int ccd_threads() { return MODE == MODE_IMPULSE ? kCcdThreads : 2 * kCcdThreads; }
+template <int MODE> constexpr int ccd_threads() { return kCcdThreads; }   // one wavefront per workgroup in both modes
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
 // tiles and a tile adds at most 3 * kCcdThreads.
 constexpr int kFlush = kCcdThreads / 2;
@@ -1133,72 +1133,64 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
     return p;
 }
 
-// One tile of kCcdThreads items runs in four block phases, each with one record per thread,
-// so that lanes of a wave follow the same branch:
-//   A  item -> tetrahedron -> coefficients -> cubic_prepare; the record (item and the
-//      doubles its branch's root formulas read) is written binned by branch to LDS;
-//   B  cubic_finish on the binned records; roots inside [0, 1] go to an LDS root list
-//      (record index, root);
-//   C  filterCollisions per root record (the tetrahedron is gathered again);
+// One tile of kCcdThreads items per step, in four phases.  Every workgroup of this kernel is
+// one wavefront, so its list counters are wave-uniform registers (a lane's place in a list
+// comes from a ballot) and the LDS exchanges between phases need only wave-level ordering
+// (wave_sync), not a workgroup barrier, whose fence would also wait for the wave's
+// outstanding global loads and atomics.
+//   A  item -> tetrahedron -> coefficients -> cubic_prepare, in registers;
+//   B  the lane's roots inside [0, 1] (at most three) go to an LDS root list (item, root),
+//      placed by a wave-wide scan of the counts;
+//   C  filterCollisions per root in whole rounds of kCcdThreads roots (the tetrahedron is
+//      gathered again); fewer than a round are carried to the next tile;
 //   D  (impulse mode) accepted collisions wait in an LDS stage; kFlush at a time they are
-//      scattered through an LDS hash table that reuses the record and root-list memory:
-//      one thread per collision computes its impulse parameters (gathering the
-//      tetrahedron once for the weights and once for the rest, so that this step needs
-//      no more registers than the CCD phases), then one thread per corner adds to the
-//      table.  The scatter stays in this kernel: run as a kernel of its own over
-//      per-workgroup collision lists it took 14 ms per pass on the 130k cloth against ~6
-//      here, where other workgroups' fp64 work hides its atomic latency.
-// Each thread's item of the next tile is loaded while this tile is solved.  Occupancy (one-
-// wave blocks): the collect mode 90 VGPRs and 9.3 KB of LDS per block (17 blocks per CU),
-// the impulse mode four waves per SIMD (128 VGPRs, 8.7 KB: sixteen blocks per CU).  At five
-// waves the impulse mode spills about a hundred registers and ran 40% slower; at three 13%
-// slower.  Staging each wave's
-// query triangle in LDS (so that half the corner reads leave the vector memory path) made
-// the impulse mode spill and took 196-209 ms per step instead of 177.
-struct SolveRec {
-    int64_t item;
-    double v[4];    // quadratic: b c d; triple: a d; trig / Cardano: a b g h
-};
+//      scattered through an LDS hash table that reuses the root-list memory: one lane per
+//      collision computes its impulse parameters (gathering the tetrahedron once for the
+//      weights and once for the rest, so that this step needs no more registers than the
+//      CCD phases), then one lane per corner adds to the table.  The scatter stays in this
+//      kernel: run as a kernel of its own over per-workgroup collision lists it took 14 ms
+//      per pass on the 130k cloth against ~6 here, where other workgroups' fp64 work hides
+//      its atomic latency.
+//      (collect mode) accepted rows are written kCollectFlush at a time with one
+//      reservation (the row counter is one contended device address).
+// Each lane's item of the next tile is loaded while this tile is solved.  The cubic's
+// branches diverge within the wave whatever the order of the items, so the items are not
+// grouped by branch.  Occupancy: the impulse mode four waves per SIMD (128 VGPRs).  At five
+// waves the impulse mode spilled about a hundred registers and ran 40% slower; at three 13%
+// slower.  Staging each wave's query triangle in LDS (so that half the corner reads leave
+// the vector memory path) made the impulse mode spill and took 196-209 ms per step
+// instead of 177.
 constexpr int kCollectFlush = 64;
 
-__device__ __forceinline__ void rec_store(int code, const CubicState &st, SolveRec &r) {
-    if (code == CUBIC_QUADRATIC) { r.v[0] = st.b; r.v[1] = st.c; r.v[2] = st.d; r.v[3] = 0; }
-    else if (code == CUBIC_TRIPLE) { r.v[0] = st.a; r.v[1] = st.d; r.v[2] = 0; r.v[3] = 0; }
-    else { r.v[0] = st.a; r.v[1] = st.b; r.v[2] = st.g; r.v[3] = st.h; }
-}
-
-__device__ __forceinline__ CubicState rec_load(int code, const SolveRec &r) {
-    CubicState st{0, 0, 0, 0, 0, 0, 0};
-    if (code == CUBIC_QUADRATIC) { st.b = r.v[0]; st.c = r.v[1]; st.d = r.v[2]; }
-    else if (code == CUBIC_TRIPLE) { st.a = r.v[0]; st.d = r.v[1]; }
-    else { st.a = r.v[0]; st.b = r.v[1]; st.g = r.v[2]; st.h = r.v[3]; }
-    return st;
+// Orders the wave's LDS accesses before this point against those after it.  The lanes of a
+// wavefront run in lockstep and the LDS serves one wave's accesses in issue order, so no
+// instruction is needed: the fences and the barrier only keep the compiler from moving
+// memory accesses across the point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <typename T, int MODE, bool DET>
-__global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
                                                   T *__restrict__ dv_all, T *__restrict__ dx_all, int *__restrict__ cnt_all,
                                                   long long *__restrict__ det_all, DevCounters *__restrict__ ctr_all) {
-    constexpr int NT = ccd_threads<MODE>();     // items per tile, threads per block
+    constexpr int NT = kCcdThreads;     // items per tile: the lanes of the one wavefront
+    static_assert(NT == 64, "one wavefront per workgroup");
     constexpr int kRootCapT = 3 * NT;
-    static_assert(MODE != MODE_IMPULSE || NT == kCcdThreads, "the impulse table is walked by kCcdThreads threads");
-    // the records (phases A-C) and the root list (phases B-C) share their LDS with the
-    // impulse table of phase D
-    constexpr size_t kSolveBytes = sizeof(SolveRec) * NT + (sizeof(T) + sizeof(unsigned short)) * kRootCapT;
+    // the root list (phases B-C) shares its LDS with the impulse table of phase D
+    constexpr size_t kListBytes = (sizeof(int64_t) + sizeof(T)) * kRootCapT;
     constexpr size_t kTableBytes = DET ? DetTable::bytes() : ImpTable<T>::bytes();
-    constexpr size_t kScatterBytes = kTableBytes + sizeof(ImpulseParams<T>) * kFlush;
-    __shared__ __align__(16) unsigned char s_raw[kSolveBytes > kScatterBytes ? kSolveBytes : kScatterBytes];
-    SolveRec *s_rec = reinterpret_cast<SolveRec *>(s_raw);
-    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(SolveRec) * NT);
-    unsigned short *s_rrec = reinterpret_cast<unsigned short *>(s_raw + sizeof(SolveRec) * NT + sizeof(T) * kRootCapT);
-    __shared__ int s_cnt[5];
-    __shared__ int s_nroot;
+    constexpr size_t kScatterBytes = MODE == MODE_IMPULSE ? kTableBytes + sizeof(ImpulseParams<T>) * kFlush : 0;
+    __shared__ __align__(16) unsigned char s_raw[kListBytes > kScatterBytes ? kListBytes : kScatterBytes];
+    int64_t *s_ritem = reinterpret_cast<int64_t *>(s_raw);
+    T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(int64_t) * kRootCapT);
     // accepted collisions wait here: impulse mode until kFlush of them are scattered,
-    // collect mode until kCollectFlush of them are written as output rows with one
-    // reservation (the row counter is one contended device address)
+    // collect mode until kCollectFlush of them are written as output rows
     constexpr int kStageCap = MODE == MODE_IMPULSE ? kImpStage : kCollectFlush + 3 * NT;
     __shared__ int64_t s_item[kStageCap];
     __shared__ T s_root[kStageCap];
@@ -1206,9 +1198,6 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     // tile as (item, root) so that the filter runs full waves
     __shared__ int64_t s_citem[NT];
     __shared__ T s_croot[NT];
-    __shared__ int s_ncarry;
-    __shared__ unsigned long long s_obase;
-    __shared__ int s_fill;
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
     // edge-edge items [0, ME) from the front of the list, vertex-face items from the back
@@ -1222,26 +1211,22 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     T *dx = MODE == MODE_IMPULSE ? dx_all + (int64_t)b * n * 9 : nullptr;
     int *cnt = MODE == MODE_IMPULSE ? cnt_all + (int64_t)b * n * 3 : nullptr;
     long long *det = DET ? det_all + (int64_t)b * n * 3 * kDetWords : nullptr;
-    // block-wide counts, the same in every lane (scalar registers); lane 0 adds them at the end
+    // wave-uniform counts: roots found, collisions scattered, staged collisions, carried roots
     unsigned long long nroots = 0, ncoll = 0;
-    if (tid == 0) {
-        s_fill = 0;
-        s_ncarry = 0;
-    }
-    __syncthreads();
+    int fill = 0, ncarry = 0;
 
     // Staged collisions are scattered kFlush at a time: their corner impulses are first
     // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
     // dx / cnt with one set of global atomics.
-    static_assert(kScatterBytes <= sizeof(s_raw), "the impulse table and parameters fit the phase A-C buffers");
+    static_assert(kScatterBytes <= sizeof(s_raw), "the impulse table and parameters fit the root-list buffer");
     static_assert(kFlush <= NT && (4 * kFlush) % NT == 0, "whole rounds of corners");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
     DetTable dtab = DetTable::over(s_raw);
     ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + kTableBytes);
-    // Scatters staged collisions [0, f) in chunks of kFlush: one thread per collision
-    // computes its impulse parameters, then one thread per tetrahedron corner adds that
-    // corner's share to the LDS table (the corner step holds few registers, so the
-    // collision step alone bounds this phase's register use).
+    // Scatters staged collisions [0, f) in chunks of kFlush: one lane per collision computes
+    // its impulse parameters, then one lane per tetrahedron corner adds that corner's share
+    // to the LDS table (the corner step holds few registers, so the collision step alone
+    // bounds this phase's register use).
     auto flush = [&](int f) {
         for (int base = 0; base < f; base += kFlush) {
             const int nf = f - base < kFlush ? f - base : kFlush;
@@ -1264,7 +1249,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
                 }
                 s_imp[tid] = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
             }
-            __syncthreads();
+            wave_sync();
             for (int t = tid; t < 4 * kFlush; t += NT) {
                 const int k = t / kFlush, jc = t % kFlush;
                 if (jc < nf) {
@@ -1286,19 +1271,19 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
             if (DET) dtab.drain(tid, det, cnt);
             else tab.drain(tid, dv, dx, cnt);
-            __syncthreads();
+            wave_sync();
         }
     };
 
     // Collect mode: the staged collisions [0, f) to output rows (rows past out_cap are
-    // counted, not written).  Every thread calls it.
+    // counted, not written).
     auto write_rows = [&](int f) {
-        if (tid == 0) s_obase = f ? atomicAdd(&ctr->coll, (unsigned long long)f) : 0ull;
-        __syncthreads();
-        const unsigned long long ob = s_obase;
+        unsigned long long ob = 0;
+        if (tid == 0 && f) ob = atomicAdd(&ctr->coll, (unsigned long long)f);
+        ob = readfirstlane_u64(ob);
         int64_t *coll = coll_all + (int64_t)b * out_cap * 3;
         for (int j = tid; j < f; j += NT) {
             const unsigned long long slot = ob + (unsigned long long)j;
@@ -1310,26 +1295,32 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
                 roots_all[(int64_t)b * out_cap + slot] = s_root[j];
             }
         }
-        __syncthreads();
+        wave_sync();
     };
 
-    // Filter of one root (filterCollisions); accepted collisions join the stage.
-    auto filter_one = [&](int64_t rit, T root) {
-        int64_t x, y;
-        int type;
-        unpack_item(rit, x, y, type);
-        const MTet<T> m = gather_mtet_rec(opaque_ptr(tv), x, y, type);
-        if (filter_collision(m, type, root)) {
-            const int p = wave_append1(&s_fill);
+    // Filter of one root per lane where `on` (filterCollisions); accepted collisions join the
+    // stage in lane order.  Every lane calls it.
+    auto filter_round = [&](bool on, int64_t rit, T root) {
+        bool acc = false;
+        if (on) {
+            int64_t x, y;
+            int type;
+            unpack_item(rit, x, y, type);
+            acc = filter_collision(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root);
+        }
+        const unsigned long long m = __ballot(acc);
+        if (acc) {
+            const int p = fill + (int)lanes_below(m);
             s_item[p] = rit;
             s_root[p] = root;
         }
+        fill += (int)__popcll(m);
     };
 
-    // The tiles of CCC_FOR_TILES, walked by hand so that each thread's item of the next
-    // tile is loaded one tile ahead (after this tile's gathers are issued); -1: no item.
-    // Tiles [0, tilesE) hold edge-edge items, the rest vertex-face items, so a tile's roots
-    // and collisions take one branch of the filter and of the impulse in the whole wave.
+    // The tiles of CCC_FOR_TILES, walked by hand so that each lane's item of the next tile is
+    // loaded one tile ahead (after this tile's gathers are issued); -1: no item.  Tiles
+    // [0, tilesE) hold edge-edge items, the rest vertex-face items, so a tile's roots and
+    // collisions take one branch of the filter and of the impulse in the whole wave.
     const int64_t tilesE = (ME + NT - 1) / NT;
     const int64_t ntiles = tilesE + (MV + NT - 1) / NT;
     const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
@@ -1346,7 +1337,7 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
     };
     int64_t next_item = item_of(blockIdx.x / kXcds);
     for (int64_t tk = blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
-        // ---- A: coefficients and branch of one item per thread
+        // ---- A: coefficients and branch of the lane's item
         const int64_t it = next_item;
         int code = CUBIC_NONE;
         CubicState st;
@@ -1361,106 +1352,79 @@ __global__ __launch_bounds__(ccd_threads<MODE>(), MODE == MODE_IMPULSE ? 4 : 5) 
             }
             code = cubic_prepare(cf[0], cf[1], cf[2], cf[3], st);
         }
-        if (tid < 5) s_cnt[tid] = 0;
-        if (tid == 0) s_nroot = 0;
-        __syncthreads();
-        // binned by branch: one LDS atomic per wave and branch
-        int pos = 0;
-#pragma unroll
-        for (int cc = CUBIC_QUADRATIC; cc <= CUBIC_CARDANO; ++cc)
-            if (code == cc) pos = wave_append1(&s_cnt[cc]);
-        __syncthreads();
-        const int o2 = s_cnt[CUBIC_QUADRATIC], o3 = o2 + s_cnt[CUBIC_TRIPLE], o4 = o3 + s_cnt[CUBIC_TRIG];
-        const int tot = o4 + s_cnt[CUBIC_CARDANO];
-        if (code != CUBIC_NONE) {
-            const int p = (code == CUBIC_QUADRATIC ? 0 : code == CUBIC_TRIPLE ? o2 : code == CUBIC_TRIG ? o3 : o4) + pos;
-            SolveRec r;
-            r.item = it;
-            rec_store(code, st, r);
-            s_rec[p] = r;
-        }
-        __syncthreads();
         next_item = item_of(tk + tstep);
-        // ---- B: finish the cubic on branch-binned records; roots in [0, 1] to the root list
-        if (tid < tot) {
-            const int c = tid < o2 ? CUBIC_QUADRATIC : tid < o3 ? CUBIC_TRIPLE : tid < o4 ? CUBIC_TRIG : CUBIC_CARDANO;
-            const CubicState rs = rec_load(c, s_rec[tid]);
-            cubic_roots<T>(c, rs, [&](T r) {
-                const int p = wave_append1(&s_nroot);
-                s_rrec[p] = (unsigned short)tid;
-                s_rroot[p] = r;
-            });
-        }
-        __syncthreads();
-        nroots += (unsigned long long)__builtin_amdgcn_readfirstlane(s_nroot);
-        // ---- C: filter one root per thread in whole rounds over the carried roots followed
-        // by this tile's; the rest (fewer than NT) is carried to the next tile
-        const int nr = s_nroot, nc = s_ncarry, total = nc + nr;
-        const int full = total / NT * NT;
-        for (int base = 0; base < full; base += NT) {
-            const int j = base + tid;
-            int64_t rit;
-            T root;
-            if (j < nc) {
-                rit = s_citem[j];
-                root = s_croot[j];
-            } else {
-                rit = s_rec[s_rrec[j - nc]].item;
-                root = s_rroot[j - nc];
-            }
-            filter_one(rit, root);
-        }
-        __syncthreads();      // the carried roots have been read
+        // ---- B: the lane's roots in [0, 1], in the order found, to the root list
+        T r0, r1, r2;
+        const int nk = cubic_roots<T>(code, st, r0, r1, r2);
         {
+            const unsigned long long b0 = __ballot(nk & 1), b1 = __ballot(nk & 2);
+            const int off = (int)(lanes_below(b0) + 2u * lanes_below(b1));
+            if (nk > 0) { s_ritem[off] = it; s_rroot[off] = r0; }
+            if (nk > 1) { s_ritem[off + 1] = it; s_rroot[off + 1] = r1; }
+            if (nk > 2) { s_ritem[off + 2] = it; s_rroot[off + 2] = r2; }
+            const int nr = (int)(__popcll(b0) + 2 * __popcll(b1));
+            nroots += (unsigned long long)nr;
+            wave_sync();
+            // ---- C: filter one root per lane in whole rounds over the carried roots
+            // followed by this tile's; the rest (fewer than NT) is carried to the next tile
+            const int total = ncarry + nr, full = total / NT * NT;
+            for (int base = 0; base < full; base += NT) {
+                const int j = base + tid;
+                int64_t rit;
+                T root;
+                if (j < ncarry) {
+                    rit = s_citem[j];
+                    root = s_croot[j];
+                } else {
+                    rit = s_ritem[j - ncarry];
+                    root = s_rroot[j - ncarry];
+                }
+                filter_round(true, rit, root);
+            }
+            wave_sync();      // the carried roots have been read
             // with a whole round the carry was consumed and the rest is this tile's tail;
             // without one, this tile's roots join the carry
-            const int from = full > 0 ? full - nc : 0, to = full > 0 ? 0 : nc;
+            const int from = full > 0 ? full - ncarry : 0, to = full > 0 ? 0 : ncarry;
             for (int k = tid; k < total - full - to; k += NT) {
-                s_citem[to + k] = s_rec[s_rrec[from + k]].item;
+                s_citem[to + k] = s_ritem[from + k];
                 s_croot[to + k] = s_rroot[from + k];
             }
-            if (tid == 0) s_ncarry = total - full;
+            ncarry = total - full;
+            wave_sync();      // the root list has been read (its memory holds the table)
         }
         // ---- D: scatter the staged impulses in whole chunks; the rest moves to the front
         if (MODE == MODE_IMPULSE) {
-            __syncthreads();
-            const int f = s_fill;
-            if (f >= kFlush) {
-                const int done = f - f % kFlush;
+            if (fill >= kFlush) {
+                const int done = fill - fill % kFlush;
                 flush(done);
                 ncoll += (unsigned long long)done;
-                // [done, f) and [0, f - done) do not overlap: f - done < kFlush <= done
-                if (tid < f - done) {
+                // [done, fill) and [0, fill - done) do not overlap: fill - done < kFlush <= done
+                if (tid < fill - done) {
                     s_item[tid] = s_item[done + tid];
                     s_root[tid] = s_root[done + tid];
                 }
-                __syncthreads();
-                if (tid == 0) s_fill = f - done;
+                wave_sync();
+                fill -= done;
             }
-        } else {
+        } else if (fill >= kCollectFlush) {
             // ---- D (collect): write the staged rows once kCollectFlush have gathered
-            __syncthreads();
-            const int f = s_fill;
-            if (f >= kCollectFlush) {
-                write_rows(f);
-                if (tid == 0) s_fill = 0;
-            }
+            write_rows(fill);
+            fill = 0;
         }
-        __syncthreads();
     }
     // the carried roots' last, partial round
-    __syncthreads();
-    if (tid < s_ncarry) filter_one(s_citem[tid], s_croot[tid]);
-    __syncthreads();
+    filter_round(tid < ncarry, s_citem[tid], s_croot[tid]);
+    wave_sync();
     if (MODE == MODE_IMPULSE) {
-        const int f = s_fill;
-        flush(f);
-        ncoll += (unsigned long long)f;
+        flush(fill);
+        ncoll += (unsigned long long)fill;
     } else {
-        write_rows(s_fill);
+        write_rows(fill);
     }
-    block_add(&ctr->roots, tid == 0 ? nroots : 0ull);
-    if (MODE == MODE_IMPULSE) block_add(&ctr->coll, tid == 0 ? ncoll : 0ull);
+    if (tid == 0) {
+        if (nroots) atomicAdd(&ctr->roots, nroots);
+        if (MODE == MODE_IMPULSE && ncoll) atomicAdd(&ctr->coll, ncoll);
+    }
 }
 
 // compute_impulses_from_collisions (collisions_impulses.cu:1736-1880) over given rows;
