@@ -68,17 +68,25 @@ template <uint64_t B> __device__ __forceinline__ double sgpr_const() {
 template <class TAB, int K> __device__ __forceinline__ double coef() {
     return sgpr_const<__builtin_bit_cast(uint64_t, TAB::c[K])>();
 }
+// fma(a, b, c) with the addend c in an SGPR pair, as one VOP3 v_fma_f64 (the same correctly
+// rounded fused multiply-add).  Left to itself the compiler forms v_fmac_f64, whose addend is
+// also its destination, and first copies c into VGPRs: two moves per polynomial step.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
 // p <- fma(p, t, c[k]) for k = K, K-1, ..., 0
 template <class TAB, int K> __device__ __forceinline__ double horner(double p, double t) {
     if constexpr (K < 0) return p;
-    else return horner<TAB, K - 1>(fma(p, t, coef<TAB, K>()), t);
+    else return horner<TAB, K - 1>(fma_sc(p, t, coef<TAB, K>()), t);
 }
 
 // Estrin's scheme for c[0] + c[1] t + ... + c[DEG] t^DEG, the same operations in the same
 // order as the CPU restatement: pairs c[2i] + c[2i+1] t, then pairs of pairs with t^2,
 // t^4, t^8 (each the square of the previous), an unpaired last term carried up unchanged.
 template <class TAB, int I, int DEG> __device__ __forceinline__ double estrin_pair(double t) {
-    if constexpr (I + 1 <= DEG) return fma(coef<TAB, I + 1>(), t, coef<TAB, I>());
+    if constexpr (I + 1 <= DEG) return fma_sc(coef<TAB, I + 1>(), t, coef<TAB, I>());
     else return coef<TAB, I>();
 }
 template <class TAB, int DEG> __device__ __forceinline__ double estrin(double t) {
@@ -110,16 +118,22 @@ template <class TAB, int DEG> __device__ __forceinline__ double estrin(double t)
     return q[0];
 }
 
+// Written without branches (zero, infinities and NaN are computed on a stand-in and then
+// replaced by x; the scale is a select of constants), so that a wave runs it as one straight
+// sequence.
 __device__ __forceinline__ double pm_cbrt(double x) {
-    if (x == 0.0 || !isfinite(x)) return x;
-    const double ax = fabs(x);
+    const bool special = x == 0.0 || !isfinite(x);
+    const double ax = special ? 1.0 : fabs(x);
     int e;
     const double m0 = frexp(ax, &e);
     int q = e / 3, r = e - 3 * q;
-    if (r < 0) { r += 3; q -= 1; }
+    const bool neg = r < 0;
+    r += neg ? 3 : 0;
+    q -= neg ? 1 : 0;
     double w = horner<RcbrtCoef, 3>(coef<RcbrtCoef, 4>(), m0);
-    w = w * (r == 0 ? 1.0 : r == 1 ? sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE1)>()
-                                   : sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE2)>());
+    const double s1 = sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE1)>();
+    const double s2 = sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE2)>();
+    w = w * (r == 0 ? 1.0 : r == 1 ? s1 : s2);
     const double m = ldexp(m0, r);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -134,7 +148,8 @@ __device__ __forceinline__ double pm_cbrt(double x) {
     const double res = (y3 - m) + (y3e + y2e * y);
     y = fma(-res, w2 * CCC_THIRD, y);
     y = ldexp(y, q);
-    return x < 0 ? -y : y;
+    y = x < 0 ? -y : y;
+    return special ? x : y;
 }
 
 __device__ __forceinline__ float pm_cbrtf(float x) { return (float)pm_cbrt((double)x); }
@@ -147,22 +162,21 @@ __device__ __forceinline__ double pm_asin_small(double z) {
 // |x| <= 1/2: pi/2 - asin(x); otherwise 2 asin(sqrt((1 - |x|) / 2)), reflected about pi/2
 // for x < 0.  The one asin_small evaluation takes the argument of the lane's case, so that
 // a wave holding both cases runs the polynomial once.
+// Written without branches: both double-double subtractions are formed and the lane's
+// result selected (NaN and |x| > 1 give NaN, x itself for NaN).
 __device__ __forceinline__ double pm_acos(double x) {
-    if (x != x) return x;
-    if (x > 1.0 || x < -1.0) return NAN;
     const double ax = fabs(x);
     const bool small = ax <= 0.5;
     const double a = pm_asin_small(small ? x : sqrt((1.0 - ax) * 0.5));
-    if (small) {
-        const double r = CCC_PIO2_HI - a;
-        const double err = (CCC_PIO2_HI - r) - a;
-        return r + (err + CCC_PIO2_LO);
-    }
+    const double rs = CCC_PIO2_HI - a;
+    const double es = (CCC_PIO2_HI - rs) - a;
+    const double vs = rs + (es + CCC_PIO2_LO);
     const double s = 2.0 * a;
-    if (x > 0) return s;
-    const double r = CCC_PI_HI - s;
-    const double err = (CCC_PI_HI - r) - s;
-    return r + (err + CCC_PI_LO);
+    const double rl = CCC_PI_HI - s;
+    const double el = (CCC_PI_HI - rl) - s;
+    const double vl = rl + (el + CCC_PI_LO);
+    const double v = small ? vs : x > 0 ? s : vl;
+    return x != x ? x : (x > 1.0 || x < -1.0) ? (double)NAN : v;
 }
 
 __device__ __forceinline__ double pm_sin(double y) {

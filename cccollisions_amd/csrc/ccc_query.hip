@@ -1427,6 +1427,29 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     }
 }
 
+// The narrow phase's device arithmetic on given inputs, for tests: what 0-4 evaluates the
+// portable cbrt / acos / sin / cos / cbrtf of ccc_libm.h at x[i]; what 5 runs cubic_prepare +
+// cubic_roots<double> on the coefficients c3 c2 c1 c0 = x[4i..4i+3] and writes the number of
+// roots kept in [0, 1] and those roots to y[4i..4i+3].
+__global__ void k_device_math(int what, const double *__restrict__ x, double *__restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (what == 5) {
+        CubicState st;
+        const int code = cubic_prepare(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3], st);
+        double r0, r1, r2;
+        const int k = cubic_roots<double>(code, st, r0, r1, r2);
+        y[4 * i] = (double)k;
+        y[4 * i + 1] = k > 0 ? r0 : 0.0;
+        y[4 * i + 2] = k > 1 ? r1 : 0.0;
+        y[4 * i + 3] = k > 2 ? r2 : 0.0;
+        return;
+    }
+    const double v = x[i];
+    y[i] = what == 0 ? pm_cbrt(v) : what == 1 ? pm_acos(v) : what == 2 ? pm_sin(v) : what == 3 ? pm_cos(v)
+                                                                                            : (double)pm_cbrtf((float)v);
+}
+
 // compute_impulses_from_collisions (collisions_impulses.cu:1736-1880) over given rows;
 // rows with a negative or out-of-range index are skipped.
 template <typename T>
@@ -2397,3 +2420,16 @@ ccc_status ccc_compute_impulses_from_collisions(ccc_context *ctx, const int64_t 
 }
 
 }  // extern "C"
+
+extern "C" ccc_status ccc_debug_device_math(ccc_context *ctx, int what, const double *x, double *y, int64_t n,
+                                            void *stream) {
+    ccc::clear_error();
+    if (!ctx) return ccc::bad_arg("context is NULL");
+    if (what < 0 || what > 5) return ccc::bad_arg("what must be 0-5");
+    if (n < 0 || (n > 0 && (!x || !y))) return ccc::bad_arg("x and y must be device arrays of n (what 5: 4 n) doubles");
+    if (n == 0) return CCC_OK;
+    CCC_HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(ccc::k_device_math, dim3(ccc::grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, what, x, y, n);
+    CCC_LAUNCH_CHECK("k_device_math");
+    return CCC_OK;
+}

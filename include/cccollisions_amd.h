@@ -103,6 +103,14 @@ ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs);
 #define CCC_MORTON_63 2
 ccc_status ccc_context_set_morton(ccc_context *ctx, int mode);
 
+/* Test hook (no counterpart in the reference): the narrow phase's device arithmetic on n
+ * given inputs, so that tests can compare it with the CPU oracle directly.  `x` and `y` are
+ * device arrays.  what 0-4: y[i] = cbrt / acos / sin / cos / cbrtf(float) of x[i], the
+ * portable functions the cubic solver uses; what 5: x[4i..4i+3] are the cubic coefficients
+ * c3 c2 c1 c0 (src/collisions_continuous.cu:511-613 solveCubic), y[4i] the number of roots
+ * kept in [0, 1] and y[4i+1..4i+3] those roots in order (0 where absent). */
+ccc_status ccc_debug_device_math(ccc_context *ctx, int what, const double *x, double *y, int64_t n, void *stream);
+
 /* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,
  * every kernel launch of a call is bracketed by HIP events on the call's stream and the
  * elapsed times are accumulated per kernel name.  Enabling or disabling clears the totals. */
