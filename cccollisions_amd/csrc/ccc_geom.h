@@ -347,8 +347,10 @@ __device__ __forceinline__ int cubic_candidates(int code, const CubicState &st, 
         }
     } else if (code == CUBIC_TRIPLE) {
         // :564-575 triple root through a float k
+        // k >= 0 ? -cbrtf(k) : cbrtf(-k), with one cube root
         float k = (float)(d / a);
-        x0 = k >= 0 ? (double)(pm_cbrtf(k) * -1) : (double)pm_cbrtf(-k);
+        const float ck = pm_cbrtf(k >= 0 ? k : -k);
+        x0 = k >= 0 ? (double)(ck * -1) : (double)ck;
         return 1;
     } else if (code == CUBIC_TRIG || code == CUBIC_CARDANO) {
         const bool trig = code == CUBIC_TRIG;
@@ -372,7 +374,9 @@ __device__ __forceinline__ int cubic_candidates(int code, const CubicState &st, 
         // :592-611 one real root
         const double S = R >= 0 ? cb : -cb;
         const double V = -(g / 2.0) - sq;
-        const double U = V >= 0 ? pm_cbrt(V) : -pm_cbrt(-V);
+        // V >= 0 ? cbrt(V) : -cbrt(-V), with one cube root (the same bits, -0 and NaN too)
+        const double cv = pm_cbrt(V >= 0 ? V : -V);
+        const double U = V >= 0 ? cv : -cv;
         x0 = (S + U) - b3a;
         return 1;
     }
