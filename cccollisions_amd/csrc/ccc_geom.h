@@ -148,16 +148,17 @@ __device__ __forceinline__ MTet<T> gather_mtet(const T *t0, const T *t1, int64_t
     return m;
 }
 
-// Vertex records of the narrow phase: for each triangle corner its position at the start
-// and at the end of the step, [x0 y0 z0 x1 y1 z1] (48 B for double, 16-byte aligned), so
-// that one corner is one contiguous read (three 16-byte loads) instead of two 24-byte reads
-// from the t0 and t1 soups.
+// Vertex records of the narrow phase: for each triangle corner its position at the start of
+// the step and its motion over the step, [x0 y0 z0 dx dy dz] with d = x1 - x0 (the moving
+// tetrahedron's velocity, figures.cu:74-96, rounded once here instead of at every read),
+// 48 B for double, 16-byte aligned, so that one corner is one contiguous read (three
+// 16-byte loads) instead of two 24-byte reads from the t0 and t1 soups.
 template <typename T> struct alignas(2 * sizeof(T)) VRec { T a[6]; };
 
 template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64_t corner, V3<T> &v, V3<T> &dv) {
     const VRec<T> r = reinterpret_cast<const VRec<T> *>(tv)[corner];
     v = mk(r.a[0], r.a[1], r.a[2]);
-    dv = mk(r.a[3], r.a[4], r.a[5]) - v;
+    dv = mk(r.a[3], r.a[4], r.a[5]);
 }
 
 // Sub-primitive filter of a moving pair (collisions_continuous.cu:234-318, threshold 0): the
@@ -178,7 +179,7 @@ template <typename T> __device__ __forceinline__ uint32_t moving_mask_axes(const
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const T *r = (k < 3 ? rx : ry) + 6 * (k % 3);
-            const T v = r[d], w = v + (r[3 + d] - v);
+            const T v = r[d], w = v + r[3 + d];
             v0[k] = v;
             lo[k] = tmin(v, w);
             hi[k] = tmax(v, w);

@@ -13,9 +13,12 @@
 //          y <- y - (y^3 - m) w^2 / 3 with the residual y^3 - m formed exactly by fma.
 //   acos : |x| <= 1/2: pi/2 - asin(x); |x| > 1/2: 2 asin(sqrt((1-|x|)/2)) (mirrored with
 //          pi for x < 0); asin(z) = z + z^3 P(z^2), P of degree 13 on [0, 1/4] evaluated
-//          by Estrin's scheme (short dependency chains); pi and pi/2 as double-double.
+//          by Horner's scheme; pi and pi/2 as double-double.
 //   sin, cos : sin(y) = y + y^3 S(y^2), cos(y) = 1 + y^2 C(y^2), S and C of degree 6 and 7
-//          on y^2 in [0, 1.1], Estrin's scheme (the solver's arguments lie in [0, pi/3]).
+//          on y^2 in [0, 1.1], Horner's scheme (the solver's arguments lie in [0, pi/3]).
+// Horner rather than Estrin: each step is one fma reading its coefficient from SGPRs, while
+// Estrin's pairs of two coefficients need one of them copied to VGPRs first; the kernels are
+// bound by VALU instruction issue, not by the longer dependency chain.
 // The polynomials are Chebyshev interpolants rounded to double (gen_libm_coeffs.py prints
 // them), materialised in SGPRs at their point of use.
 #pragma once
@@ -82,42 +85,6 @@ template <class TAB, int K> __device__ __forceinline__ double horner(double p, d
     else return horner<TAB, K - 1>(fma_sc(p, t, coef<TAB, K>()), t);
 }
 
-// Estrin's scheme for c[0] + c[1] t + ... + c[DEG] t^DEG, the same operations in the same
-// order as the CPU restatement: pairs c[2i] + c[2i+1] t, then pairs of pairs with t^2,
-// t^4, t^8 (each the square of the previous), an unpaired last term carried up unchanged.
-template <class TAB, int I, int DEG> __device__ __forceinline__ double estrin_pair(double t) {
-    if constexpr (I + 1 <= DEG) return fma_sc(coef<TAB, I + 1>(), t, coef<TAB, I>());
-    else return coef<TAB, I>();
-}
-template <class TAB, int DEG> __device__ __forceinline__ double estrin(double t) {
-    static_assert(DEG >= 4 && DEG <= 15, "four combination levels at most");
-    constexpr int M0 = DEG / 2 + 1;                 // pairs at level 0
-    double q[8];
-    q[0] = estrin_pair<TAB, 0, DEG>(t);
-    q[1] = estrin_pair<TAB, 2, DEG>(t);
-    q[2] = estrin_pair<TAB, 4, DEG>(t);
-    if constexpr (M0 > 3) q[3] = estrin_pair<TAB, 6, DEG>(t);
-    if constexpr (M0 > 4) q[4] = estrin_pair<TAB, 8, DEG>(t);
-    if constexpr (M0 > 5) q[5] = estrin_pair<TAB, 10, DEG>(t);
-    if constexpr (M0 > 6) q[6] = estrin_pair<TAB, 12, DEG>(t);
-    if constexpr (M0 > 7) q[7] = estrin_pair<TAB, 14, DEG>(t);
-    double tp = t * t;
-    int m = M0;
-#pragma unroll
-    for (int level = 0; level < 3; ++level) {
-        if (m <= 1) break;
-        int k = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-            if (i >= m) break;
-            q[k++] = i + 1 < m ? fma(q[i + 1], tp, q[i]) : q[i];
-        }
-        m = k;
-        tp = tp * tp;
-    }
-    return q[0];
-}
-
 // Written without branches (zero, infinities and NaN are computed on a stand-in and then
 // replaced by x; the scale is a select of constants), so that a wave runs it as one straight
 // sequence.
@@ -156,7 +123,7 @@ __device__ __forceinline__ float pm_cbrtf(float x) { return (float)pm_cbrt((doub
 
 __device__ __forceinline__ double pm_asin_small(double z) {
     const double t = z * z;
-    return fma(z * t, estrin<AsinCoef, 13>(t), z);
+    return fma(z * t, horner<AsinCoef, 12>(coef<AsinCoef, 13>(), t), z);
 }
 
 // |x| <= 1/2: pi/2 - asin(x); otherwise 2 asin(sqrt((1 - |x|) / 2)), reflected about pi/2
@@ -181,12 +148,12 @@ __device__ __forceinline__ double pm_acos(double x) {
 
 __device__ __forceinline__ double pm_sin(double y) {
     const double t = y * y;
-    return fma(y * t, estrin<SinCoef, 6>(t), y);
+    return fma(y * t, horner<SinCoef, 5>(coef<SinCoef, 6>(), t), y);
 }
 
 __device__ __forceinline__ double pm_cos(double y) {
     const double t = y * y;
-    return fma(t, estrin<CosCoef, 7>(t), 1.0);
+    return fma(t, horner<CosCoef, 6>(coef<CosCoef, 7>(), t), 1.0);
 }
 
 }  // namespace ccc

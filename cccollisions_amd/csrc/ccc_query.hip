@@ -748,7 +748,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     int64_t K = (int64_t)(kc < (unsigned long long)capK ? kc : (unsigned long long)capK);
     if (K > c1) K = c1;     // this launch covers the pairs [c0, c1)
     const int2 *pairs = pairs_all + (int64_t)b * capK;
-    // SUB_STATIC: the t0 soup; otherwise the vertex records [x0 y0 z0 x1 y1 z1] per corner
+    // SUB_STATIC: the t0 soup; otherwise the vertex records [x0 y0 z0 dx dy dz] per corner
     const T *src = src_all + (int64_t)b * n * (KIND == SUB_STATIC ? 9 : 18);
     int64_t *out = out_all + (int64_t)b * capM * (KIND == SUB_MINCOL ? 3 : 1);
     constexpr int kTile = PP * kThreads;
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
 //            with atomics into dv / dx / cnt (collisions_impulses.cu:1312-1366).
 // The tetrahedron is gathered again for the filter so that it is not live across the
 // cubic solver (fewer VGPRs, more waves per SIMD).
-// Vertex records [x0 y0 z0 x1 y1 z1] per triangle corner from the t0 / t1 soups.
+// Vertex records [x0 y0 z0 dx dy dz] (d = x1 - x0) per triangle corner from the t0 / t1 soups.
 template <typename T>
 __global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restrict__ t1_all, int64_t corners,
                                  T *__restrict__ tv_all) {
@@ -862,7 +862,7 @@ __global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restri
     const T *a = t0_all + (b * corners + i) * 3, *c = t1_all + (b * corners + i) * 3;
     VRec<T> r;
     r.a[0] = a[0]; r.a[1] = a[1]; r.a[2] = a[2];
-    r.a[3] = c[0]; r.a[4] = c[1]; r.a[5] = c[2];
+    r.a[3] = c[0] - a[0]; r.a[4] = c[1] - a[1]; r.a[5] = c[2] - a[2];
     reinterpret_cast<VRec<T> *>(tv_all)[b * corners + i] = r;
 }
 
