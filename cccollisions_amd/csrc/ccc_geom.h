@@ -215,16 +215,36 @@ template <typename T> __device__ __forceinline__ uint32_t moving_mask_axes(const
     return eq ? 0u : m;
 }
 
-// gather_mtet from vertex records (same values, so the same tetrahedron bit for bit)
+// gather_mtet from vertex records (same values, so the same tetrahedron bit for bit).  The
+// record of each tetrahedron corner is picked as type_corner does, with the edge-edge and
+// the vertex-face types on separate paths: the narrow phase's waves hold one kind, so a wave
+// runs one of the two short address computations.
 template <typename T>
 __device__ __forceinline__ MTet<T> gather_mtet_rec(const T *tv, int64_t x, int64_t y, int type) {
+    // record indices (corner 3 * triangle + i, below 3 * 2^28 as meshes hold fewer than 2^28
+    // triangles); integers rather than pointers, so that the loads keep the records' global
+    // address space
+    const uint32_t rx = 3u * (uint32_t)x, ry = 3u * (uint32_t)y;
+    uint32_t c[4];
+    if (type < 9) {
+        // edge type / 3 of x, edge type % 3 of y
+        const int ea = type / 3, eb = type - 3 * ea;
+        c[0] = rx + edge_first(ea);
+        c[1] = rx + edge_second(ea);
+        c[2] = ry + edge_first(eb);
+        c[3] = ry + edge_second(eb);
+    } else {
+        // 9-11: face x, vertex type - 9 of y; 12-14: vertex type - 12 of x, face y
+        const bool inv = type >= 12;
+        const uint32_t j = (uint32_t)(inv ? type - 12 : type - 9);
+        c[0] = inv ? rx + j : rx;
+        c[1] = inv ? ry : rx + 1;
+        c[2] = inv ? ry + 1 : rx + 2;
+        c[3] = inv ? ry + 2 : ry + j;
+    }
     MTet<T> m;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int side, idx;
-        type_corner(type, k, side, idx);
-        ld_vrec(tv, 3 * (side ? y : x) + idx, m.v[k], m.dv[k]);
-    }
+    for (int k = 0; k < 4; ++k) ld_vrec(tv, c[k], m.v[k], m.dv[k]);
     return m;
 }
 

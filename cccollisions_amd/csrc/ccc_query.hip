@@ -1126,11 +1126,14 @@ struct DetTable {
     }
 };
 
-// Hides a pointer's provenance from the optimiser so that loads through it are not
-// merged with earlier loads of the same addresses.
+// The same pointer, offset by a zero the optimiser cannot see through, so that loads
+// through it are not merged with earlier loads of the same addresses.  The pointer itself
+// keeps its provenance (a kernel argument), so its loads stay global-memory loads rather
+// than flat ones.
 template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    int64_t z;
+    asm volatile("s_mov_b64 %0, 0" : "=s"(z));
+    return p + z;
 }
 
 // One tile of kCcdThreads items per step, in four phases.  Every workgroup of this kernel is
