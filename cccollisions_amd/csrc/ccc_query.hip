@@ -667,10 +667,10 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 // u' * kThreads + t' < u * kThreads + t.  SUB_MOVING keeps two such runs per chunk, the
 // edge-edge items from the front of the list (counted in `items`) and the vertex-face items
 // from its back (item j at capM - 1 - j), both counted in `items_split` (edge-edge in the low
-// and vertex-face in the high 32 bits), so that the narrow phase's tiles hold one kind.  The moving masks are computed one axis at a time
-// from the vertex records (moving_mask_axes), which keeps the registers low enough for PP
-// pairs per thread.  A tile's items are assembled in LDS (a 2-byte entry per item: pair in
-// tile and type) and written as one contiguous run.
+// and vertex-face in the high 32 bits), so that the narrow phase's tiles hold one kind.  The
+// moving masks come from the vertex records (moving_mask_axes).  A tile's items are
+// assembled in LDS (a 2-byte entry per item: pair in tile and type) and written as one
+// contiguous run per kind.
 enum { SUB_MOVING = 0, SUB_STATIC = 1, SUB_MINCOL = 2 };
 // Exclusive block-wide scan of cnt (< 16) over the NT threads in thread order, without a
 // reservation: the thread's offset; the block's sum in `total`.  Every thread calls it.
