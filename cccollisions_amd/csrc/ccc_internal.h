@@ -151,14 +151,12 @@ enum : unsigned { DEV_BAD_TREE = 1u, DEV_STACK = 2u, DEV_UNORDERED = 4u };
 // Per-batch-element counters kept on the device during a call.
 struct alignas(64) DevCounters {
     unsigned long long cand;    // K (total, may exceed the candidate capacity)
-    unsigned long long items;   // M (the moving narrow phase: its edge-edge items)
+    unsigned long long items;   // M of the current pair chunk
     unsigned long long roots;   // V
     unsigned long long coll;    // R (total, may exceed the output capacity)
     unsigned long long items_done;  // M of the pair chunks already finished
-    unsigned long long items_split; // the moving narrow phase's items of the current chunk,
-                                    // (vertex-face << 32) | edge-edge
     unsigned int flags;         // DEV_* bits
-    unsigned int pad[3];
+    unsigned int pad[5];
 };
 static_assert(sizeof(DevCounters) == 64, "DevCounters is one 64-byte line");
 

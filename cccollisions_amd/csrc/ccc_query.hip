@@ -664,13 +664,13 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
 //   SUB_STATIC: boxes grown by the threshold (proximity_static.cu:11-118) -> packed items;
 //   SUB_MINCOL: swept boxes, written as (x, y, type) rows (find_minimal_candidates, :321-338).
 // Items stay in pair order: pair u * kThreads + t of a tile comes after every pair
-// u' * kThreads + t' < u * kThreads + t.  SUB_MOVING keeps two such runs per chunk, the
-// edge-edge items from the front of the list (counted in `items`) and the vertex-face items
-// from its back (item j at capM - 1 - j), both counted in `items_split` (edge-edge in the low
-// and vertex-face in the high 32 bits), so that the narrow phase's tiles hold one kind.  The
-// moving masks come from the vertex records (moving_mask_axes).  A tile's items are
-// assembled in LDS (a 2-byte entry per item: pair in tile and type) and written as one
-// contiguous run per kind.
+// u' * kThreads + t' < u * kThreads + t.  SUB_MOVING writes a tile's edge-edge items first and
+// its vertex-face items after them, each in pair order, so that the narrow phase's 64-item
+// tiles hold one kind except where the kind changes (about 3% of them on the 130k cloth),
+// while a pair's items of both kinds stay close together in the list (their vertex records
+// are read again while still in the caches).  The moving masks come from the vertex records
+// (moving_mask_axes).  A tile's items are assembled in LDS (a 2-byte entry per item: pair in
+// tile and type) and written as one contiguous run.
 enum { SUB_MOVING = 0, SUB_STATIC = 1, SUB_MINCOL = 2 };
 // Exclusive block-wide scan of cnt (< 16) over the NT threads in thread order, without a
 // reservation: the thread's offset; the block's sum in `total`.  Every thread calls it.
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     constexpr bool SPLIT = KIND == SUB_MOVING;
     __shared__ unsigned short s_ent[kEnt];
     __shared__ int2 s_pair[PP * kThreads];
-    __shared__ unsigned long long s_base, s_base_vf;
+    __shared__ unsigned long long s_base;
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
     const unsigned long long kc = ctr->cand;
@@ -803,15 +803,8 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
             }
         }
         if (tid == 0) {
-            if (SPLIT) {
-                // one reservation for both runs (the counter is one contended address)
-                const unsigned long long add = (unsigned long long)tot | ((unsigned long long)tot_vf << 32);
-                const unsigned long long old = add ? atomicAdd(&ctr->items_split, add) : 0ull;
-                s_base = old & 0xffffffffull;
-                s_base_vf = old >> 32;
-            } else {
-                s_base = tot ? atomicAdd(&ctr->items, (unsigned long long)tot) : 0ull;
-            }
+            const unsigned all = tot + tot_vf;
+            s_base = all ? atomicAdd(&ctr->items, (unsigned long long)all) : 0ull;
         }
         __syncthreads();
         const unsigned long long base = s_base;
@@ -828,14 +821,14 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
             }
         }
         if (SPLIT) {
-            // vertex-face item j of the chunk sits at capM - 1 - j
-            const unsigned long long bv = s_base_vf;
+            // vertex-face item j of the tile follows its tot edge-edge items
+            const unsigned long long bv = base + tot;
             const unsigned long long rv = bv < (unsigned long long)capM ? (unsigned long long)capM - bv : 0ull;
             const unsigned limv = rv < tot_vf ? (unsigned)rv : tot_vf;
             for (unsigned k = tid; k < limv; k += kThreads) {
                 const unsigned e = s_ent[kEnt - 1 - k];
                 const int2 pr = s_pair[e >> 4];
-                out[capM - 1 - (int64_t)(bv + k)] = pack_item(pr.x, pr.y, (int)(e & 15u));
+                out[bv + k] = pack_item(pr.x, pr.y, (int)(e & 15u));
             }
         }
         __syncthreads();
@@ -1203,10 +1196,8 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     __shared__ T s_croot[NT];
     const int b = blockIdx.y, tid = threadIdx.x;
     DevCounters *ctr = ctr_all + b;
-    // edge-edge items [0, ME) from the front of the list, vertex-face items from the back
-    const unsigned long long ms = ctr->items_split, mc = ms & 0xffffffffull, mv = ms >> 32;
-    const int64_t ME = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
-    const int64_t MV = (int64_t)(mv < (unsigned long long)(capM - ME) ? mv : (unsigned long long)(capM - ME));
+    const unsigned long long mc = ctr->items;
+    const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
     const int64_t *items = items_all + (int64_t)b * capM;
     const T *tv = tv_all + (int64_t)b * n * 18;
     const T *masses = MODE == MODE_IMPULSE ? masses_all + (int64_t)b * n * 3 : nullptr;
@@ -1321,22 +1312,16 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     };
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each lane's item of the next tile is
-    // loaded one tile ahead (after this tile's gathers are issued); -1: no item.  Tiles
-    // [0, tilesE) hold edge-edge items, the rest vertex-face items, so a tile's roots and
-    // collisions take one branch of the filter and of the impulse in the whole wave.
-    const int64_t tilesE = (ME + NT - 1) / NT;
-    const int64_t ntiles = tilesE + (MV + NT - 1) / NT;
+    // loaded one tile ahead (after this tile's gathers are issued); -1: no item.  The moving
+    // items come in runs of one kind (edge-edge, then vertex-face, per k_subprim tile), so
+    // nearly every tile's roots and collisions take one branch of the filter and of the
+    // impulse in the whole wave.
+    const int64_t ntiles = (M + NT - 1) / NT;
     const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
     const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
     auto item_of = [&](int64_t k) -> int64_t {
-        const int64_t t = tbase + k;
-        if (k >= tper || t >= ntiles) return -1;
-        if (t < tilesE) {
-            const int64_t i = t * NT + tid;
-            return i < ME ? items[i] : (int64_t)-1;
-        }
-        const int64_t j = (t - tilesE) * NT + tid;
-        return j < MV ? items[capM - 1 - j] : (int64_t)-1;
+        const int64_t i = (tbase + k) * NT + tid;
+        return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
     };
     int64_t next_item = item_of(blockIdx.x / kXcds);
     for (int64_t tk = blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
@@ -1695,10 +1680,8 @@ struct Query {
 // End of a pair chunk: its sub-primitive items are done; the next chunk refills the list.
 __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
     for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
-        const unsigned long long sp = ctr[b].items_split;
-        ctr[b].items_done += ctr[b].items + (sp & 0xffffffffull) + (sp >> 32);
+        ctr[b].items_done += ctr[b].items;
         ctr[b].items = 0;
-        ctr[b].items_split = 0;
     }
 }
 
@@ -1706,8 +1689,7 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 // another size): the item list holds at most 15 items per pair of a chunk, 4 GiB at this
 // size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
-// Largest chunk: the moving narrow phase counts a chunk's edge-edge items (at most 9 per
-// pair) in 32 bits.
+// Largest chunk: at most 15 * 2^28 items (32 GiB) per batch element.
 constexpr int64_t kMaxPairChunk = (int64_t)1 << 28;
 static int64_t pair_chunk_of(const ccc_context *ctx, int64_t chunk_ext, int64_t pairs, int64_t B) {
     const int64_t budget = chunk_ext > 0 ? chunk_ext * B : ctx->pair_chunk > 0 ? ctx->pair_chunk : kPairChunk;
