@@ -860,12 +860,11 @@ __global__ void k_vertex_records(const T *__restrict__ t0_all, const T *__restri
 }
 
 enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
-// Threads (items) per k_ccd tile: one wave per block.  The block phases wait at barriers for
-// their slowest wave, so smaller blocks waste less of that wait (k_ccd per step on the 130k
-// cloth: 4 waves per block 206 ms, 2 waves 193 then 175, 1 wave 172.5).
+// Threads (items) per k_ccd tile, in both modes: one wave per block, so that its phases need
+// no workgroup barrier (k_ccd per step on the 130k cloth: 4 waves per block 206 ms, 2 waves
+// 193 then 175, 1 wave 172.5 with barriers; the collect call 20.8 -> 17.1 ms once its tiles
+// were one wave too).
 constexpr int kCcdThreads = 64;
-// The collect mode keeps two-wave tiles: its output stage needs fewer registers than the
-// scatter, and one-wave tiles measured slower there (20.0 against 18.3 ms per pass).
 template <int MODE> constexpr int ccd_threads() { return kCcdThreads; }   // one wavefront per workgroup in both modes
 // Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
 // tiles and a tile adds at most 3 * kCcdThreads.
