@@ -335,16 +335,17 @@ __device__ __forceinline__ int cubic_prepare(double a, double b, double c, doubl
     st.a = a; st.b = b; st.c = c; st.d = d; st.f = 0; st.g = 0; st.h = 0;
     if (a == 0 && b == 0) return CUBIC_NONE;   // :532-536 linear root computed, never stored
     if (fabs(a) < 1e-16) return CUBIC_QUADRATIC;
-    const double f = ((3.0 * c / a) - ((b * b) / (a * a))) / 3.0;
+    // the divisions by 3 and 27 give the division's bits (div_exact)
+    const double f = div_exact<3>((3.0 * c / a) - ((b * b) / (a * a)));
     const double b3 = 2 * dd_cube_lead(b);
     const double a2 = a * a;
     const double a3 = dd_cube_lead(a);
     const double p1 = b3 / a3;
     const double p2 = (b * c * 9) / a2;
     const double p3 = 27.0 * d / a;
-    const double g = (p1 - p2 + p3) / 27.0;
+    const double g = div_exact<27>(p1 - p2 + p3);
     const double f3 = dd_cube_lead(f);
-    const double h = (g * g) / 4.0 + f3 / 27.0;
+    const double h = (g * g) / 4.0 + div_exact<27>(f3);
     st.f = f; st.g = g; st.h = h;
     if (f == 0 && g == 0 && h == 0) return CUBIC_TRIPLE;
     return h <= 0 ? CUBIC_TRIG : CUBIC_CARDANO;
@@ -385,8 +386,9 @@ __device__ __forceinline__ int cubic_candidates(int code, const CubicState &st, 
             const double k = pm_acos(-(g / (2 * sq)));
             const double L = cb * -1;
             // cos(k / 3) appears twice in the formulas; evaluated once (the same value)
-            const double M = pm_cos(k / 3.0);
-            const double N = sqrt(3.0) * pm_sin(k / 3.0);
+            const double k3 = div_exact_finite<3>(k);      // k in [0, pi] or NaN
+            const double M = pm_cos(k3);
+            const double N = sqrt(3.0) * pm_sin(k3);
             const double P = b3a * -1;
             x0 = 2 * cb * M - b3a;
             x1 = L * (M + N) + P;

@@ -108,7 +108,9 @@ ccc_status ccc_context_set_morton(ccc_context *ctx, int mode);
  * device arrays.  what 0-4: y[i] = cbrt / acos / sin / cos / cbrtf(float) of x[i], the
  * portable functions the cubic solver uses; what 5: x[4i..4i+3] are the cubic coefficients
  * c3 c2 c1 c0 (src/collisions_continuous.cu:511-613 solveCubic), y[4i] the number of roots
- * kept in [0, 1] and y[4i+1..4i+3] those roots in order (0 where absent). */
+ * kept in [0, 1] and y[4i+1..4i+3] those roots in order (0 where absent); what 6 / 7 / 8:
+ * y[i] = x[i] / 3, x[i] / 27 and x[i] / 3 for finite or NaN x[i], by the solver's
+ * division-free sequences (equal to the division bit for bit). */
 ccc_status ccc_debug_device_math(ccc_context *ctx, int what, const double *x, double *y, int64_t n, void *stream);
 
 /* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,

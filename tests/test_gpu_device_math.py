@@ -114,8 +114,43 @@ def test_cubic_roots_bit_exact(dmath, oracle):
         assert _same_bits(got[i, 1:1 + k], want), (c, got[i], want)
 
 
+def _division_inputs(rng, n):
+    """Doubles of every exponent and sign (random bit patterns), values near 1, subnormals,
+    mantissas of all ones at every exponent, and the special values."""
+    u = rng.integers(0, 2**63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    anyb = u.view(np.float64)
+    near1 = rng.uniform(-4, 4, n)
+    sub = (u & np.uint64(0x800FFFFFFFFFFFFF)).view(np.float64)
+    e = np.arange(2047, dtype=np.uint64)
+    tops = ((e << np.uint64(52)) | np.uint64(0xFFFFFFFFFFFFF)).view(np.float64)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, -5e-324, 1.7976931348623157e308, 3.0, 27.0, 1.0])
+    return np.concatenate([anyb, near1, sub, tops, -tops, special])
+
+
+@pytest.mark.parametrize("what,div", [(6, 3.0), (7, 27.0)])
+def test_exact_constant_division(dmath, what, div):
+    """The solver's division-free x / 3 and x / 27 give the division's bits on every input
+    class, NaN payloads included."""
+    _, _, run = dmath
+    x = _division_inputs(np.random.default_rng(13 + what), 200000)
+    got = run(what, x)
+    want = x / div
+    bad = np.nonzero(got.view(np.uint64) != want.view(np.uint64))[0]
+    assert bad.size == 0, (x[bad[:5]], got[bad[:5]], want[bad[:5]])
+
+
+def test_exact_division_by_three_on_angles(dmath):
+    """The guard-free variant, as the trigonometric form uses it (angles in [0, pi], NaN)."""
+    _, _, run = dmath
+    rng = np.random.default_rng(17)
+    x = np.concatenate([rng.uniform(0, np.pi, 200000), 10.0 ** rng.uniform(-300, 0, 20000), [0.0, np.pi, np.nan]])
+    got = run(8, x)
+    want = x / 3.0
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
 def test_hook_rejects_bad_arguments(dmath):
     lib, ctx, _ = dmath
-    assert lib.ccc_debug_device_math(ctx, 6, None, None, 0, None) != 0
+    assert lib.ccc_debug_device_math(ctx, 9, None, None, 0, None) != 0
     assert lib.ccc_debug_device_math(ctx, 0, None, None, 4, None) != 0
     assert lib.ccc_debug_device_math(ctx, 0, None, None, 0, None) == 0
