@@ -866,9 +866,11 @@ enum { MODE_COLLECT = 0, MODE_IMPULSE = 1 };
 // were one wave too).
 constexpr int kCcdThreads = 64;
 template <int MODE> constexpr int ccd_threads() { return kCcdThreads; }   // one wavefront per workgroup in both modes
-// Staged collisions are scattered kFlush at a time; fewer than kFlush stay staged between
-// tiles and a tile adds at most 3 * kCcdThreads.
-constexpr int kFlush = kCcdThreads / 2;
+// Staged collisions are scattered kFlush at a time, one lane each for their impulse
+// parameters and in two halves of kFlush / 2 for the corner sums; fewer than kFlush stay
+// staged between tiles and a tile adds at most 3 * kCcdThreads.
+constexpr int kFlush = kCcdThreads;
+constexpr int kHalfFlush = kFlush / 2;
 constexpr int kImpStage = kFlush + 3 * kCcdThreads;
 
 template <typename T>
@@ -969,8 +971,11 @@ __global__ void k_det_finalize(const long long *__restrict__ acc, int64_t corner
 
 // Per-block sums of corner impulses: open addressing on the corner key 3 * triangle +
 // corner, kImpSlots slots (a power of two), linear probing up to kImpProbe slots; a
-// contribution that finds no slot goes to the global arrays directly.
-constexpr int kImpSlots = 2 * kFlush;
+// contribution that finds no slot goes to the global arrays directly.  A half flush adds
+// 4 kHalfFlush = 128 corner shares, on fewer distinct corners (neighbouring collisions share
+// corners); a whole flush at once overfilled the table (every share that found no slot
+// spent the whole probe window first): k_ccd 16.2 -> 23.4 ms per pass on the 130k cloth.
+constexpr int kImpSlots = 64;
 constexpr int kImpProbe = 16;
 constexpr int log2_exact(int v) { return v <= 1 ? 0 : 1 + log2_exact(v / 2); }
 constexpr int kImpBits = log2_exact(kImpSlots);
@@ -1142,7 +1147,7 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //      scattered through an LDS hash table that reuses the root-list memory: one lane per
 //      collision computes its impulse parameters (gathering the tetrahedron once for the
 //      weights and once for the rest, so that this step needs no more registers than the
-//      CCD phases), then one lane per corner adds to the table.  The scatter stays in this
+//      CCD phases), then adds its four corners' shares to the table.  The scatter stays in this
 //      kernel: run as a kernel of its own over per-workgroup collision lists it took 14 ms
 //      per pass on the 130k cloth against ~6 here, where other workgroups' fp64 work hides
 //      its atomic latency.
@@ -1180,7 +1185,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     // the root list (phases B-C) shares its LDS with the impulse table of phase D
     constexpr size_t kListBytes = (sizeof(int64_t) + sizeof(T)) * kRootCapT;
     constexpr size_t kTableBytes = DET ? DetTable::bytes() : ImpTable<T>::bytes();
-    constexpr size_t kScatterBytes = MODE == MODE_IMPULSE ? kTableBytes + sizeof(ImpulseParams<T>) * kFlush : 0;
+    constexpr size_t kScatterBytes = MODE == MODE_IMPULSE ? kTableBytes : 0;
     __shared__ __align__(16) unsigned char s_raw[kListBytes > kScatterBytes ? kListBytes : kScatterBytes];
     int64_t *s_ritem = reinterpret_cast<int64_t *>(s_raw);
     T *s_rroot = reinterpret_cast<T *>(s_raw + sizeof(int64_t) * kRootCapT);
@@ -1211,20 +1216,20 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     // Staged collisions are scattered kFlush at a time: their corner impulses are first
     // summed per corner in an LDS hash table, then each occupied slot adds its sums to dv /
     // dx / cnt with one set of global atomics.
-    static_assert(kScatterBytes <= sizeof(s_raw), "the impulse table and parameters fit the root-list buffer");
-    static_assert(kFlush <= NT && (4 * kFlush) % NT == 0, "whole rounds of corners");
+    static_assert(kScatterBytes <= sizeof(s_raw), "the impulse table fits the root-list buffer");
+    static_assert(kFlush == NT && kHalfFlush == 32, "one lane per collision of a flush; halves of 32 lanes");
     ImpTable<T> tab = ImpTable<T>::over(s_raw);
     DetTable dtab = DetTable::over(s_raw);
-    ImpulseParams<T> *s_imp = reinterpret_cast<ImpulseParams<T> *>(s_raw + kTableBytes);
-    // Scatters staged collisions [0, f) in chunks of kFlush: one lane per collision computes
-    // its impulse parameters, then one lane per tetrahedron corner adds that corner's share
-    // to the LDS table (the corner step holds few registers, so the collision step alone
-    // bounds this phase's register use).
+    // Scatters staged collisions [0, f) in chunks of kFlush.  One lane per collision computes
+    // its impulse parameters; the parameters of lanes l and l ^ 32 are then exchanged, and
+    // each half of the chunk is summed in the LDS table and drained: collision j of the half
+    // has corners 0 and 2 added by lane j and corners 1 and 3 by lane j + 32 (the corner
+    // steps hold few registers, so the parameter step alone bounds this phase's register
+    // use).
     auto flush = [&](int f) {
         for (int base = 0; base < f; base += kFlush) {
             const int nf = f - base < kFlush ? f - base : kFlush;
-            if (DET) dtab.clear(tid);
-            else tab.clear(tid);
+            ImpulseParams<T> ip{};
             if (tid < nf) {
                 int64_t x, y;
                 int type;
@@ -1240,34 +1245,57 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
                     type_corner(type, k, side, idx);
                     q[k] = masses[3 * (side ? y : x) + idx];
                 }
-                s_imp[tid] = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
+                ip = impulse_params(gather_mtet_rec(opaque_ptr(tv), x, y, type), type, root, w, q);
             }
-            wave_sync();
-            for (int t = tid; t < 4 * kFlush; t += NT) {
-                const int k = t / kFlush, jc = t % kFlush;
-                if (jc < nf) {
+            ImpulseParams<T> pp;      // lane l ^ 32's parameters
+            pp.nrm.x = __shfl_xor(ip.nrm.x, 32);
+            pp.nrm.y = __shfl_xor(ip.nrm.y, 32);
+            pp.nrm.z = __shfl_xor(ip.nrm.z, 32);
+            pp.im0 = __shfl_xor(ip.im0, 32);
+            pp.im1 = __shfl_xor(ip.im1, 32);
+            pp.keep = __shfl_xor(ip.keep, 32);
+            const int j = tid & (kHalfFlush - 1), hi = tid >> 5;
+            for (int h = 0; h * kHalfFlush < nf; ++h) {
+                const int nh = nf - h * kHalfFlush < kHalfFlush ? nf - h * kHalfFlush : kHalfFlush;
+                if (DET) dtab.clear(tid);
+                else tab.clear(tid);
+                wave_sync();
+                if (j < nh) {
+                    // collision h * 32 + j: this lane's own when hi == h, else lane l ^ 32's
+                    const bool own = hi == h;
+                    ImpulseParams<T> cp;
+                    cp.nrm.x = own ? ip.nrm.x : pp.nrm.x;
+                    cp.nrm.y = own ? ip.nrm.y : pp.nrm.y;
+                    cp.nrm.z = own ? ip.nrm.z : pp.nrm.z;
+                    cp.im0 = own ? ip.im0 : pp.im0;
+                    cp.im1 = own ? ip.im1 : pp.im1;
+                    cp.keep = own ? ip.keep : pp.keep;
                     int64_t x, y;
                     int type;
-                    unpack_item(s_item[base + jc], x, y, type);
-                    int side, idx;
-                    type_corner(type, k, side, idx);
-                    const int64_t c = 3 * (side ? y : x) + idx;
-                    V3<T> v, dvk;
-                    ld_vrec(opaque_ptr(tv), c, v, dvk);
-                    V3<T> imp, imd;
-                    corner_impulse(s_imp[jc], type, k, dvk, imp, imd);
-                    if (DET) {
-                        const double v[6] = {(double)imp.x, (double)imp.y, (double)imp.z, (double)imd.x, (double)imd.y, (double)imd.z};
-                        dtab.add(c, v, det, cnt);
-                    } else {
-                        tab.add(c, imp, imd, dv, dx, cnt);
+                    unpack_item(s_item[base + h * kHalfFlush + j], x, y, type);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        const int k = 2 * r + hi;
+                        int side, idx;
+                        type_corner(type, k, side, idx);
+                        const int64_t c = 3 * (side ? y : x) + idx;
+                        V3<T> v, dvk;
+                        ld_vrec(opaque_ptr(tv), c, v, dvk);
+                        V3<T> imp, imd;
+                        corner_impulse(cp, type, k, dvk, imp, imd);
+                        if (DET) {
+                            const double v[6] = {(double)imp.x, (double)imp.y, (double)imp.z, (double)imd.x, (double)imd.y, (double)imd.z};
+                            dtab.add(c, v, det, cnt);
+                        } else {
+                            tab.add(c, imp, imd, dv, dx, cnt);
+                        }
                     }
                 }
+                wave_sync();
+                if (DET) dtab.drain(tid, det, cnt);
+                else tab.drain(tid, dv, dx, cnt);
+                wave_sync();
             }
-            wave_sync();
-            if (DET) dtab.drain(tid, det, cnt);
-            else tab.drain(tid, dv, dx, cnt);
-            wave_sync();
         }
     };
 
