@@ -456,6 +456,37 @@ __device__ __forceinline__ int ratio_side(float n, float d) {
     return 0;
 }
 
+// Whether s = RN(n / d) passes the edge-edge filter's test !(s < 0 || s > 1) -- s in [0, 1],
+// -0 included, or NaN -- for d = |c|^2 (+0 or more, or NaN), decided without dividing, with
+// the division's answer.  For finite n and d in [dlo, max] (d u exact, u = 2^-53 / 2^-24):
+// s >= 0 iff n >= 0 or n / d rounds to -0, i.e. |n| <= d * 2^-1075 / 2^-150 (ties go to the
+// even -0), tested as ldexp(|n|, 1075 / 150) <= d (exact, or +inf); s <= 1 iff
+// n / d <= 1 + u (ties go to the even 1), i.e. n - d <= d u, where n - d is exact for n in
+// [d/2, 2d] (Sterbenz) and has the deciding sign elsewhere.  d = 0: s is NaN for n = 0
+// (passes) and +-inf otherwise; d = +inf: s is +-0 or NaN (passes); n = +-inf with finite
+// d > 0: +-inf (fails); NaN n or d: NaN (passes).  Only 0 < d < dlo is left to the division
+// (the caller's `tiny`).
+template <typename T> struct UnitRatioK;
+template <> struct UnitRatioK<double> {
+    static constexpr double dlo = 0x1p-960, u = 0x1p-53;
+    static constexpr int up = 1075;
+};
+template <> struct UnitRatioK<float> {
+    static constexpr float dlo = 0x1p-100f, u = 0x1p-24f;
+    static constexpr int up = 150;
+};
+template <typename T> __device__ __forceinline__ bool unit_ratio(T n, T d) {
+    using K = UnitRatioK<T>;
+    const bool nan = n != n || d != d;
+    const bool ge0 = n >= 0 || ldexp(-n, K::up) <= d;
+    const bool le1 = n - d <= d * K::u;
+    const bool fin = ge0 && le1;                                 // d in [dlo, max], finite n
+    const bool at0 = n == 0;                                     // d = 0
+    const bool dinf = d == (T)__builtin_huge_val();
+    const bool ninf = tabs(n) == (T)__builtin_huge_val();
+    return nan || dinf || (d == 0 ? at0 : !ninf && fin);
+}
+
 // collisions_continuous.cu:615-637 is_inside (NaN passes every comparison: accepted).  A
 // barycentric coordinate that is certainly outside [0, 1] rejects before any division
 // (the reference rejects too, at that check or an earlier one).
@@ -483,11 +514,14 @@ template <typename T> __device__ __forceinline__ bool filter_collision(const MTe
         V3<T> da = p[1] - p[0], db = p[3] - p[2], dc = p[2] - p[0];
         V3<T> cab = cross(da, db);
         const T ns = dot(cross(dc, db), cab), nq = dot(cross(dc, da), cab), den = norm2(cab);
-        // a parameter certainly outside [0, 1] rejects without dividing (same result)
-        if (ratio_side(ns, den) != 0 || ratio_side(nq, den) != 0) return false;
-        T s = ns / den;
-        T q = nq / den;
-        return !(s < 0 || s > 1 || q < 0 || q > 1);
+        // s = ns / den and q = nq / den in [0, 1] (or NaN), decided without dividing except
+        // for 0 < den < dlo (same result)
+        if (den > 0 && den < UnitRatioK<T>::dlo) {
+            T s = ns / den;
+            T q = nq / den;
+            return !(s < 0 || s > 1 || q < 0 || q > 1);
+        }
+        return unit_ratio(ns, den) && unit_ratio(nq, den);
     }
     // vertex-face: types 9-11 test vertex p[3] against face p[0..2], types 12-14 vertex p[0]
     // against face p[1..3]; one is_inside on the lane's corners (a wave holding both runs it

@@ -106,24 +106,39 @@ template <int D> __device__ __forceinline__ double div_exact(double x) {
     return t != t ? q : r;
 }
 
+// sqrt(x) for x = +-0, x >= 2^-767, x < 0 or NaN: the correctly rounded sequence the compiler
+// emits for sqrt (reciprocal square root estimate, then Goldschmidt / Newton steps with fma,
+// x itself for +-0 and +inf), without its rescaling of arguments below 2^-767, which is the
+// identity on these inputs.  Gives sqrt's bits there.
+__device__ __forceinline__ double sqrt_no_rescale(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return (x == 0.0 || x == __builtin_huge_val()) ? x : g;
+}
+
 // p <- fma(p, t, c[k]) for k = K, K-1, ..., 0
 template <class TAB, int K> __device__ __forceinline__ double horner(double p, double t) {
     if constexpr (K < 0) return p;
     else return horner<TAB, K - 1>(fma_sc(p, t, coef<TAB, K>()), t);
 }
 
-// Written without branches (zero, infinities and NaN are computed on a stand-in and then
-// replaced by x; the scale is a select of constants), so that a wave runs it as one straight
-// sequence.
+// Written without branches (zero, infinities and NaN run through the same steps, whose
+// result is then replaced by x; the scale is a select of constants), so that a wave runs it
+// as one straight sequence.  e = 3q + r with r in {0, 1, 2} is split as an unsigned division
+// of e + 1200 (frexp exponents lie in [-1073, 1024]), the same q and r as the floor division.
 __device__ __forceinline__ double pm_cbrt(double x) {
     const bool special = x == 0.0 || !isfinite(x);
-    const double ax = special ? 1.0 : fabs(x);
     int e;
-    const double m0 = frexp(ax, &e);
-    int q = e / 3, r = e - 3 * q;
-    const bool neg = r < 0;
-    r += neg ? 3 : 0;
-    q -= neg ? 1 : 0;
+    const double m0 = frexp(fabs(x), &e);
+    const unsigned n = (unsigned)(e + 1200), qb = n / 3u;
+    const int r = (int)(n - 3u * qb), q = (int)qb - 400;
     double w = horner<RcbrtCoef, 3>(coef<RcbrtCoef, 4>(), m0);
     const double s1 = sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE1)>();
     const double s2 = sgpr_const<__builtin_bit_cast(uint64_t, CCC_CBRT_SCALE2)>();
@@ -141,8 +156,7 @@ __device__ __forceinline__ double pm_cbrt(double x) {
     const double y3 = y2 * y, y3e = fma(y2, y, -y3);
     const double res = (y3 - m) + (y3e + y2e * y);
     y = fma(-res, w2 * CCC_THIRD, y);
-    y = ldexp(y, q);
-    y = x < 0 ? -y : y;
+    y = copysign(ldexp(y, q), x);
     return special ? x : y;
 }
 
@@ -161,7 +175,8 @@ __device__ __forceinline__ double pm_asin_small(double z) {
 __device__ __forceinline__ double pm_acos(double x) {
     const double ax = fabs(x);
     const bool small = ax <= 0.5;
-    const double a = pm_asin_small(small ? x : sqrt((1.0 - ax) * 0.5));
+    // (1 - |x|) / 2 is 0, at least 2^-54, negative (|x| > 1) or NaN
+    const double a = pm_asin_small(small ? x : sqrt_no_rescale((1.0 - ax) * 0.5));
     const double rs = CCC_PIO2_HI - a;
     const double es = (CCC_PIO2_HI - rs) - a;
     const double vs = rs + (es + CCC_PIO2_LO);

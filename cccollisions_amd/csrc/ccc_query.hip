@@ -1463,7 +1463,9 @@ __global__ void k_device_math(int what, const double *__restrict__ x, double *__
     const double v = x[i];
     y[i] = what == 0 ? pm_cbrt(v) : what == 1 ? pm_acos(v) : what == 2 ? pm_sin(v) : what == 3 ? pm_cos(v)
          : what == 4 ? (double)pm_cbrtf((float)v) : what == 6 ? div_exact<3>(v) : what == 7 ? div_exact<27>(v)
-                                                                                 : div_exact_finite<3>(v);
+         : what == 8 ? div_exact_finite<3>(v) : what == 9 ? sqrt_no_rescale(v)
+         : what == 10 ? (unit_ratio(v, x[n + i]) ? 1.0 : 0.0)
+                      : (unit_ratio((float)v, (float)x[n + i]) ? 1.0 : 0.0);
 }
 
 // compute_impulses_from_collisions (collisions_impulses.cu:1736-1880) over given rows;
@@ -2438,8 +2440,8 @@ extern "C" ccc_status ccc_debug_device_math(ccc_context *ctx, int what, const do
                                             void *stream) {
     ccc::clear_error();
     if (!ctx) return ccc::bad_arg("context is NULL");
-    if (what < 0 || what > 8) return ccc::bad_arg("what must be 0-8");
-    if (n < 0 || (n > 0 && (!x || !y))) return ccc::bad_arg("x and y must be device arrays of n (what 5: 4 n) doubles");
+    if (what < 0 || what > 11) return ccc::bad_arg("what must be 0-11");
+    if (n < 0 || (n > 0 && (!x || !y))) return ccc::bad_arg("x and y must be device arrays of n doubles (what 5: 4 n each; what 10, 11: x 2 n)");
     if (n == 0) return CCC_OK;
     CCC_HIP_TRY(hipSetDevice(ctx->device));
     hipLaunchKernelGGL(ccc::k_device_math, dim3(ccc::grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, what, x, y, n);

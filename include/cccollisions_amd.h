@@ -110,7 +110,10 @@ ccc_status ccc_context_set_morton(ccc_context *ctx, int mode);
  * c3 c2 c1 c0 (src/collisions_continuous.cu:511-613 solveCubic), y[4i] the number of roots
  * kept in [0, 1] and y[4i+1..4i+3] those roots in order (0 where absent); what 6 / 7 / 8:
  * y[i] = x[i] / 3, x[i] / 27 and x[i] / 3 for finite or NaN x[i], by the solver's
- * division-free sequences (equal to the division bit for bit). */
+ * division-free sequences (equal to the division bit for bit); what 9: sqrt(x[i]) without
+ * rescaling (x[i] = +-0, >= 2^-767, negative or NaN); what 10 / 11: y[i] = 1 when the
+ * edge-edge filter's division-free test accepts the quotient x[i] / x[n + i] in double /
+ * float (x holds 2 n values), else 0. */
 ccc_status ccc_debug_device_math(ccc_context *ctx, int what, const double *x, double *y, int64_t n, void *stream);
 
 /* Opt-in kernel timing (no counterpart in the reference; used by the benchmark).  While on,

@@ -5,7 +5,9 @@ the cubic solver (cubic_prepare + cubic_roots) on random cubics with three real 
 with one, on each branch of the reference's solveCubic (collisions_continuous.cu:511-613):
 linear, quadratic (|c3| < 1e-16, including a double root), triple root, trigonometric and
 Cardano forms, and coefficients with infinities and NaN.  The oracle's side is pinned
-against mpmath and hand cases in tests/test_oracle_ccd.py.
+against mpmath and hand cases in tests/test_oracle_ccd.py.  The kernels' replacements of
+IEEE operations -- division by 3 and 27, sqrt without rescaling, the edge-edge filter's
+division-free parameter test -- are compared with numpy's operations.
 """
 import ctypes
 import os
@@ -149,8 +151,72 @@ def test_exact_division_by_three_on_angles(dmath):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
+def test_sqrt_without_rescaling(dmath):
+    """sqrt on the arguments pm_acos gives it (0, 2^-54 and up, negative, NaN): sqrt's bits."""
+    _, _, run = dmath
+    rng = np.random.default_rng(19)
+    x = np.concatenate([10.0 ** rng.uniform(-230, 308, 100000), rng.uniform(0, 0.5, 100000),
+                        2.0 ** -54 * (1 + rng.uniform(0, 1, 1000)), [0.0, -0.0, np.inf, 2.0 ** -767, 0.25, 0.5]])
+    got = run(9, x)
+    assert np.array_equal(got.view(np.uint64), np.sqrt(x).view(np.uint64))
+    bad = run(9, np.array([-1.0, -np.inf, np.nan, -1e-300]))
+    assert np.isnan(bad).all()
+
+
+def _ratio_pairs(rng, dtype):
+    """(n, d) pairs around every decision of the edge-edge parameter test: quotients within a
+    few ulps of 1 (the 1 + u tie included) and of 0 (tiny negative n that round to -0), random
+    pairs of every exponent, and the special operands.  d is +0, NaN, +inf or at least the
+    smallest value the test decides without dividing."""
+    fin = np.finfo(dtype)
+    dlo = dtype(2.0 ** -960) if dtype == np.float64 else dtype(2.0 ** -100)
+    m = 20000
+    emax = 300 if dtype == np.float64 else 30
+    d = (10.0 ** rng.uniform(-emax, emax, m)).astype(dtype)
+    d = np.maximum(d, dlo)
+    n = []
+    dd = []
+    u = dtype(fin.eps / 2)
+    for k in (-3, -2, -1, 0, 1, 2, 3):
+        n.append((d * (dtype(1) + dtype(k) * u)).astype(dtype))
+        dd.append(d)
+    one_tie = (d.astype(np.float64) * (1 + float(u))).astype(dtype)        # RN(d (1 + u))
+    n += [one_tie, np.nextafter(one_tie, dtype(np.inf)), np.nextafter(one_tie, dtype(0))]
+    dd += [d, d, d]
+    # tiny negative numerators over large denominators: n / d rounds to -0 or to -tiny
+    tiny = fin.smallest_subnormal
+    big_d = (2.0 ** rng.integers(-2, 60, m)).astype(dtype)
+    for j in (1, 2, 3, 4, 5):
+        n.append(np.full(m, -tiny * j, dtype))
+        dd.append(big_d)
+    n.append((rng.uniform(-2, 2, m) * d).astype(dtype))
+    dd.append(d)
+    n.append((10.0 ** rng.uniform(-emax, emax, m) * rng.choice([-1, 1], m)).astype(dtype))
+    dd.append(d)
+    sp_n = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-30, -1e-30], dtype)
+    sp_d = np.array([0.0, np.inf, np.nan, 1.0, dlo, fin.max], dtype)
+    gn, gd = np.meshgrid(sp_n, sp_d)
+    n.append(gn.ravel())
+    dd.append(gd.ravel())
+    return np.concatenate(n), np.concatenate(dd)
+
+
+@pytest.mark.parametrize("what,dtype", [(10, np.float64), (11, np.float32)])
+def test_edge_edge_parameter_test_without_division(dmath, what, dtype):
+    """unit_ratio(n, d) equals !(s < 0 || s > 1) for s = n / d rounded in the dtype."""
+    _, _, run = dmath
+    n, d = _ratio_pairs(np.random.default_rng(what), dtype)
+    with np.errstate(all="ignore"):
+        s = n / d
+    want = ~((s < 0) | (s > 1))
+    # x = [n, d]: the hook reads x[i] and x[n + i] for i < n (per=2 passes n = x.size / 2)
+    got = run(what, np.concatenate([n.astype(np.float64), d.astype(np.float64)]), per=2)[: n.size] == 1.0
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, (n[bad[:5]], d[bad[:5]], s[bad[:5]], got[bad[:5]])
+
+
 def test_hook_rejects_bad_arguments(dmath):
     lib, ctx, _ = dmath
-    assert lib.ccc_debug_device_math(ctx, 9, None, None, 0, None) != 0
+    assert lib.ccc_debug_device_math(ctx, 12, None, None, 0, None) != 0
     assert lib.ccc_debug_device_math(ctx, 0, None, None, 4, None) != 0
     assert lib.ccc_debug_device_math(ctx, 0, None, None, 0, None) == 0
