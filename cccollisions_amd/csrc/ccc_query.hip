@@ -562,14 +562,6 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
         if (sp >= DEPTH || !fits) { atomicOr(&ctr->flags, DEV_STACK); alive = false; }
         else stack[sp++][tid] = (StackT)v;
     };
-    auto leaf_pair = [&](int ref) -> int {     // the pair (query, leaf ~ref) or -1
-        const LeafRec lr = recs[~ref];
-        if (overlap_inward(q, lr, thr) || overlap(q, ld_box(bb + 6 * (n - 1 + (int64_t)(~ref))), thr)) {
-            const int oid = lr.id;
-            if (!PARTIAL || !to_check[oid] || qid < oid) return oid;
-        }
-        return -1;
-    };
     while (__any(alive)) {
         int o0 = -1, o1 = -1, o2 = -1, o3 = -1;     // pairs found in this iteration
         int r0 = 0, r1 = 0, r2 = 0, r3 = 0;         // children (entries) that may be descended into
@@ -592,6 +584,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                 const float mnz[4] = {qn.mnz.x, qn.mnz.y, qn.mnz.z, qn.mnz.w}, mxz[4] = {qn.mxz.x, qn.mxz.y, qn.mxz.z, qn.mxz.w};
                 const int rf[4] = {qn.ref.x, qn.ref.y, qn.ref.z, qn.ref.w}, rmv[4] = {qn.rm.x, qn.rm.y, qn.rm.z, qn.rm.w};
                 int oid[4];
+                bool leaf[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     bool hit = qx0 <= mxx[k] && qx1 >= mnx[k] && qy0 <= mxy[k] && qy1 >= mny[k] && qz0 <= mxz[k] && qz1 >= mnz[k];
@@ -599,9 +592,38 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(const T *__restrict__
                     // entry k starts after entry k-1's last leaf; the node itself (so entry 0)
                     // is only entered when its first leaf is below lim
                     if (k > 0) hit = hit && rmv[k - 1] + 1 < lim;
-                    oid[k] = -1;
-                    if (hit && rf[k] < 0) oid[k] = leaf_pair(rf[k]);
-                    else if (hit) im |= 1u << k;
+                    leaf[k] = hit && rf[k] < 0;
+                    if (hit && rf[k] >= 0) im |= 1u << k;
+                }
+                // The four leaf records are requested before the first is tested, so that
+                // their loads are in flight together (record 0 stands in for entries that are
+                // not hit leaves: no branch around the loads).  A leaf whose inward-rounded
+                // record does not decide is tested on its exact box from `bboxes`, under a
+                // wave-uniform branch with every lane loading a valid row: with that fallback
+                // in a lane-divergent branch per leaf, the compiler lost the record's id in
+                // the lanes that took it (their pairs were dropped).
+                LeafRec lr[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lr[k] = recs[leaf[k] ? ~rf[k] : 0];
+                bool ov[4];
+                bool need = false;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    ov[k] = leaf[k] && overlap_inward(q, lr[k], thr);
+                    need = need || (leaf[k] && !ov[k]);
+                }
+                if (__any(need)) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const bool nk = leaf[k] && !ov[k];
+                        const Box<T> eb = ld_box(bb + 6 * (n - 1 + (int64_t)(nk ? ~rf[k] : 0)));
+                        ov[k] = ov[k] || (nk && overlap(q, eb, thr));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int id = lr[k].id;
+                    oid[k] = ov[k] && (!PARTIAL || !to_check[id] || qid < id) ? id : -1;
                 }
                 o0 = oid[0]; o1 = oid[1]; o2 = oid[2]; o3 = oid[3];
                 r0 = rf[0]; r1 = rf[1]; r2 = rf[2]; r3 = rf[3];
