@@ -754,7 +754,7 @@ __device__ __forceinline__ unsigned block_scan_excl2(unsigned a, unsigned b, uns
 template <typename T, int KIND, int PP>
 __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0, int64_t c1,
                                                       const T *__restrict__ src_all, int64_t n, T thr,
-                                                      int64_t *__restrict__ out_all, int64_t capM,
+                                                      void *__restrict__ out_all, int64_t capM,
                                                       DevCounters *__restrict__ ctr_all) {
     static_assert(PP >= 1 && PP * kThreads <= 4096, "pair index and type fit 16 bits");
     // (pair in tile) << 4 | type; SUB_MOVING: edge-edge entries from the front, vertex-face
@@ -772,7 +772,9 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
     const int2 *pairs = pairs_all + (int64_t)b * capK;
     // SUB_STATIC: the t0 soup; otherwise the vertex records [x0 y0 z0 dx dy dz] per corner
     const T *src = src_all + (int64_t)b * n * (KIND == SUB_STATIC ? 9 : 18);
-    int64_t *out = out_all + (int64_t)b * capM * (KIND == SUB_MINCOL ? 3 : 1);
+    // SUB_MOVING: 32-bit items (pair index within the chunk) << 4 | type; the others 64-bit
+    int64_t *out = SPLIT ? nullptr : static_cast<int64_t *>(out_all) + (int64_t)b * capM * (KIND == SUB_MINCOL ? 3 : 1);
+    uint32_t *out32 = SPLIT ? static_cast<uint32_t *>(out_all) + (int64_t)b * capM : nullptr;
     constexpr int kTile = PP * kThreads;
     const int64_t ntiles = K > c0 ? (K - c0 + kTile - 1) / kTile : 0;
     CCC_FOR_TILES(tile, ntiles) {
@@ -795,7 +797,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
                     mask = moving_mask_axes(src, x, y);
                 }
             }
-            s_pair[u * kThreads + tid] = make_int2(x, y);
+            if (!SPLIT) s_pair[u * kThreads + tid] = make_int2(x, y);
             if (SPLIT) {
                 uint32_t mee = mask & 0x1ffu, mvf = mask >> 9;
                 unsigned t;
@@ -832,14 +834,19 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
         const unsigned long long base = s_base;
         const unsigned long long room = base < (unsigned long long)capM ? (unsigned long long)capM - base : 0ull;
         const unsigned lim = room < tot ? (unsigned)room : tot;
+        const uint32_t tile_pair = (uint32_t)(tile * kTile);     // < 2^28: the chunk's pairs
         for (unsigned k = tid; k < lim; k += kThreads) {
             const unsigned e = s_ent[k];
-            const int2 pr = s_pair[e >> 4];
             const unsigned long long slot = base + k;
-            if (KIND == SUB_MINCOL) {
-                out[3 * slot] = pr.x; out[3 * slot + 1] = pr.y; out[3 * slot + 2] = (int64_t)(e & 15u);
+            if (SPLIT) {
+                out32[slot] = ((tile_pair + (e >> 4)) << 4) | (e & 15u);
             } else {
-                out[slot] = pack_item(pr.x, pr.y, (int)(e & 15u));
+                const int2 pr = s_pair[e >> 4];
+                if (KIND == SUB_MINCOL) {
+                    out[3 * slot] = pr.x; out[3 * slot + 1] = pr.y; out[3 * slot + 2] = (int64_t)(e & 15u);
+                } else {
+                    out[slot] = pack_item(pr.x, pr.y, (int)(e & 15u));
+                }
             }
         }
         if (SPLIT) {
@@ -849,8 +856,7 @@ __global__ __launch_bounds__(kThreads) void k_subprim(const int2 *__restrict__ p
             const unsigned limv = rv < tot_vf ? (unsigned)rv : tot_vf;
             for (unsigned k = tid; k < limv; k += kThreads) {
                 const unsigned e = s_ent[kEnt - 1 - k];
-                const int2 pr = s_pair[e >> 4];
-                out[bv + k] = pack_item(pr.x, pr.y, (int)(e & 15u));
+                out32[bv + k] = ((tile_pair + (e >> 4)) << 4) | (e & 15u);
             }
         }
         __syncthreads();
@@ -1195,7 +1201,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <typename T, int MODE, bool DET>
-__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const int64_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const uint32_t *__restrict__ items_all, int64_t capM,
+                                                  const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
                                                   int64_t *__restrict__ coll_all, T *__restrict__ roots_all, int64_t out_cap,
@@ -1224,7 +1231,8 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     DevCounters *ctr = ctr_all + b;
     const unsigned long long mc = ctr->items;
     const int64_t M = (int64_t)(mc < (unsigned long long)capM ? mc : (unsigned long long)capM);
-    const int64_t *items = items_all + (int64_t)b * capM;
+    const uint32_t *items = items_all + (int64_t)b * capM;
+    const int2 *pairs = pairs_all + (int64_t)b * capK + c0;     // this chunk's candidate pairs
     const T *tv = tv_all + (int64_t)b * n * 18;
     const T *masses = MODE == MODE_IMPULSE ? masses_all + (int64_t)b * n * 3 : nullptr;
     T *dv = MODE == MODE_IMPULSE ? dv_all + (int64_t)b * n * 9 : nullptr;
@@ -1361,21 +1369,34 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
     };
 
     // The tiles of CCC_FOR_TILES, walked by hand so that each lane's item of the next tile is
-    // loaded one tile ahead (after this tile's gathers are issued); -1: no item.  The moving
-    // items come in runs of one kind (edge-edge, then vertex-face, per k_subprim tile), so
-    // nearly every tile's roots and collisions take one branch of the filter and of the
-    // impulse in the whole wave.
-    const int64_t ntiles = (M + NT - 1) / NT;
-    const int64_t tper = (ntiles + kXcds - 1) / kXcds, tstep = gridDim.x / kXcds;
-    const int64_t tbase = (int64_t)(blockIdx.x % kXcds) * tper;
-    auto item_of = [&](int64_t k) -> int64_t {
-        const int64_t i = (tbase + k) * NT + tid;
-        return (k < tper && tbase + k < ntiles && i < M) ? items[i] : (int64_t)-1;
+    // loaded one tile ahead (after this tile's gathers are issued) and its candidate pair
+    // after phase B; kNoItem: no item.  An item is (pair index within the chunk) << 4 | type;
+    // the packed (x, y, type) form is made when the tile starts.  The moving items come in
+    // runs of one kind (edge-edge, then vertex-face, per k_subprim tile), so nearly every
+    // tile's roots and collisions take one branch of the filter and of the impulse in the
+    // whole wave.
+    constexpr uint32_t kNoItem = 0xffffffffu;   // type 15 does not exist
+    // tile counts in 32 bits (at most 15 * 2^28 / 64 tiles): the kernel's uniform values fill
+    // the scalar registers
+    const int ntiles = (int)((M + NT - 1) / NT);
+    const int tper = (ntiles + kXcds - 1) / kXcds, tstep = (int)gridDim.x / kXcds;
+    const int tbase = (int)(blockIdx.x % kXcds) * tper;
+    auto item_of = [&](int k) -> uint32_t {
+        const int64_t i = (int64_t)(tbase + k) * NT + tid;
+        return (k < tper && tbase + k < ntiles && i < M) ? items[i] : kNoItem;
     };
-    int64_t next_item = item_of(blockIdx.x / kXcds);
-    for (int64_t tk = blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
+    // the next tile's item waits as its pair (registers) and its type (LDS, 15: no item), so
+    // that it holds no more registers than the packed item through phases C and D
+    __shared__ unsigned char s_ntype[NT];
+    __shared__ int64_t s_myitem[NT];
+    uint32_t next_item = item_of(blockIdx.x / kXcds);
+    int2 next_pair = make_int2(0, 0);
+    if (next_item != kNoItem) next_pair = pairs[next_item >> 4];
+    s_ntype[tid] = (unsigned char)(next_item & 15u);
+    for (int tk = (int)blockIdx.x / kXcds; tk < tper && tbase + tk < ntiles; tk += tstep) {
         // ---- A: coefficients and branch of the lane's item
-        const int64_t it = next_item;
+        const int ntype = s_ntype[tid];
+        const int64_t it = ntype != 15 ? pack_item(next_pair.x, next_pair.y, ntype) : (int64_t)-1;
         int code = CUBIC_NONE;
         CubicState st;
         if (it >= 0) {
@@ -1389,6 +1410,7 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
             }
             code = cubic_prepare(cf[0], cf[1], cf[2], cf[3], st);
         }
+        s_myitem[tid] = it;     // read back for the root list, not held through the solver
         next_item = item_of(tk + tstep);
         // ---- B: the lane's roots in [0, 1], in the order found, to the root list
         T r0, r1, r2;
@@ -1396,12 +1418,19 @@ __global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_c
         {
             const unsigned long long b0 = __ballot(nk & 1), b1 = __ballot(nk & 2);
             const int off = (int)(lanes_below(b0) + 2u * lanes_below(b1));
-            if (nk > 0) { s_ritem[off] = it; s_rroot[off] = r0; }
-            if (nk > 1) { s_ritem[off + 1] = it; s_rroot[off + 1] = r1; }
-            if (nk > 2) { s_ritem[off + 2] = it; s_rroot[off + 2] = r2; }
+            // the item as parked in LDS before the solver (the empty asm keeps the compiler
+            // from holding it in registers instead: two more VGPRs spilled the solver)
+            asm volatile("" ::: "memory");
+            const int64_t itb = s_myitem[tid];
+            if (nk > 0) { s_ritem[off] = itb; s_rroot[off] = r0; }
+            if (nk > 1) { s_ritem[off + 1] = itb; s_rroot[off + 1] = r1; }
+            if (nk > 2) { s_ritem[off + 2] = itb; s_rroot[off + 2] = r2; }
             const int nr = (int)(__popcll(b0) + 2 * __popcll(b1));
             nroots += (unsigned long long)nr;
             wave_sync();
+            // the next tile's candidate pair (its item arrived during phase B)
+            if (next_item != kNoItem) next_pair = pairs[next_item >> 4];
+            s_ntype[tid] = (unsigned char)(next_item & 15u);
             // ---- C: filter one root per lane in whole rounds over the carried roots
             // followed by this tile's; the rest (fewer than NT) is carried to the next tile
             const int total = ncarry + nr, full = total / NT * NT;
@@ -1741,8 +1770,13 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 // another size): the item list holds at most 15 items per pair of a chunk, 4 GiB at this
 // size.
 constexpr int64_t kPairChunk = (int64_t)1 << 25;
-// Largest chunk: at most 15 * 2^28 items (32 GiB) per batch element.
+// Largest chunk: at most 15 * 2^28 items per batch element; the continuous narrow phase's
+// 32-bit items hold the pair index within the chunk in 28 bits.
 constexpr int64_t kMaxPairChunk = (int64_t)1 << 28;
+// Bytes per sub-primitive item: the continuous narrow phase keeps (pair index within the
+// chunk) << 4 | type in 32 bits (chunks hold at most 2^28 pairs); the static one (x, y, type)
+// in 64.
+static size_t item_bytes(Narrow nw) { return nw == Narrow::PROXIMITY ? sizeof(int64_t) : sizeof(uint32_t); }
 static int64_t pair_chunk_of(const ccc_context *ctx, int64_t chunk_ext, int64_t pairs, int64_t B) {
     const int64_t budget = chunk_ext > 0 ? chunk_ext * B : ctx->pair_chunk > 0 ? ctx->pair_chunk : kPairChunk;
     return std::max<int64_t>(std::min<int64_t>(std::min<int64_t>(pairs, budget / B), kMaxPairChunk), 1);
@@ -1940,7 +1974,7 @@ static ccc_status run_query(const Query &q) {
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
         if (ws.arena()) {
             // the item list takes what the arena has left (results do not depend on the chunk)
-            const int64_t fit = (int64_t)(ws.arena_left() / ((size_t)B * 15 * sizeof(int64_t)));
+            const int64_t fit = (int64_t)(ws.arena_left() / ((size_t)B * 15 * item_bytes(q.narrow)));
             if (fit < 1) {
                 set_error("the scratch buffer has no room left for the sub-primitive items of one candidate pair");
                 return CCC_ERR_NO_MEMORY;
@@ -1948,7 +1982,7 @@ static ccc_status run_query(const Query &q) {
             chunk = std::min(chunk, fit);
         }
         const int64_t capM = 15 * chunk;
-        int64_t *items = ws.get_as<int64_t>(B * capM);
+        void *items = ws.get_as<unsigned char>(B * capM * (int64_t)item_bytes(q.narrow));
         const unsigned gS = persistent_grid(q.ctx, (chunk + kSubprimPairs - 1) / kSubprimPairs, B);
         const unsigned gM = persistent_grid(q.ctx, capM, B);
         const unsigned gCcdC = persistent_grid(q.ctx, capM, B, ccd_threads<MODE_COLLECT>());
@@ -1977,18 +2011,18 @@ static ccc_status run_query(const Query &q) {
                 CCC_LAUNCH_CHECK("k_subprim");
                 tm.begin(q.narrow == Narrow::PROXIMITY ? "proximity" : q.narrow == Narrow::CCD_COLLECT ? "ccd_collect" : "ccd_impulse");
                 if (q.narrow == Narrow::PROXIMITY)
-                    hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, items, capM, (const T *)q.t0, n, thr,
+                    hipLaunchKernelGGL((k_proximity<T>), dim3(gM, B), dim3(kThreads), 0, s, (const int64_t *)items, capM, (const T *)q.t0, n, thr,
                                        q.out_rows, q.out_cap, ctr);
                 else if (q.narrow == Narrow::CCD_COLLECT)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT, false>), dim3(gCcdC, B), dim3(ccd_threads<MODE_COLLECT>()), 0, s, items, capM,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_COLLECT, false>), dim3(gCcdC, B), dim3(ccd_threads<MODE_COLLECT>()), 0, s, (const uint32_t *)items, capM, (const int2 *)pairs, capK, c0,
                                        (const T *)tv, (const T *)nullptr, n, q.out_rows, (T *)q.out_roots, q.out_cap,
                                        (T *)nullptr, (T *)nullptr, (int *)nullptr, (long long *)nullptr, ctr);
                 else if (q.det_acc)
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, true>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, true>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, (const uint32_t *)items, capM, (const int2 *)pairs, capK, c0,
                                        (const T *)tv, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
                                        (T *)q.dv, (T *)q.dx, q.cnt, q.det_acc, ctr);
                 else
-                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, false>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, items, capM,
+                    hipLaunchKernelGGL((k_ccd<T, MODE_IMPULSE, false>), dim3(gCcdI, B), dim3(ccd_threads<MODE_IMPULSE>()), 0, s, (const uint32_t *)items, capM, (const int2 *)pairs, capK, c0,
                                        (const T *)tv, (const T *)q.masses, n, (int64_t *)nullptr, (T *)nullptr, (int64_t)0,
                                        (T *)q.dv, (T *)q.dx, q.cnt, (long long *)nullptr, ctr);
                 tm.end();
@@ -2042,7 +2076,7 @@ static size_t resident_bytes(const Query &q) {
     const int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
     if (!q.pairs_ext) tot += r(sizeof(int2) * B * capK);
     const int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, capK, B);
-    tot += r(esz * B * n * 18) + r(sizeof(int64_t) * B * 15 * chunk);
+    tot += r(esz * B * n * 18) + r(item_bytes(q.narrow) * B * 15 * chunk);
     return tot;
 }
 
