@@ -1767,9 +1767,10 @@ __global__ void k_fold_items(DevCounters *__restrict__ ctr, int64_t B) {
 }
 
 // Pairs per narrow-phase chunk over all batch elements of a call (the context may set
-// another size): the item list holds at most 15 items per pair of a chunk, 4 GiB at this
-// size.
-constexpr int64_t kPairChunk = (int64_t)1 << 25;
+// another size) for 4-byte items, half as many for 8-byte ones: the item list holds at most
+// 15 items per pair of a chunk, about 4 GiB either way.  On the 130k cloth one chunk of 2^26
+// instead of two of 2^25 took k_ccd from 16.32 to 16.17 ms per pass.
+constexpr int64_t kPairChunk = (int64_t)1 << 26;
 // Largest chunk: at most 15 * 2^28 items per batch element; the continuous narrow phase's
 // 32-bit items hold the pair index within the chunk in 28 bits.
 constexpr int64_t kMaxPairChunk = (int64_t)1 << 28;
@@ -1777,8 +1778,9 @@ constexpr int64_t kMaxPairChunk = (int64_t)1 << 28;
 // chunk) << 4 | type in 32 bits (chunks hold at most 2^28 pairs); the static one (x, y, type)
 // in 64.
 static size_t item_bytes(Narrow nw) { return nw == Narrow::PROXIMITY ? sizeof(int64_t) : sizeof(uint32_t); }
-static int64_t pair_chunk_of(const ccc_context *ctx, int64_t chunk_ext, int64_t pairs, int64_t B) {
-    const int64_t budget = chunk_ext > 0 ? chunk_ext * B : ctx->pair_chunk > 0 ? ctx->pair_chunk : kPairChunk;
+static int64_t pair_chunk_of(const ccc_context *ctx, int64_t chunk_ext, int64_t pairs, int64_t B, size_t item_size) {
+    const int64_t dflt = kPairChunk * (int64_t)sizeof(uint32_t) / (int64_t)item_size;
+    const int64_t budget = chunk_ext > 0 ? chunk_ext * B : ctx->pair_chunk > 0 ? ctx->pair_chunk : dflt;
     return std::max<int64_t>(std::min<int64_t>(std::min<int64_t>(pairs, budget / B), kMaxPairChunk), 1);
 }
 constexpr int kSubprimPairs = 2;   // pairs per thread of k_subprim
@@ -1968,7 +1970,7 @@ static ccc_status run_query(const Query &q) {
         // sub-primitive items and the narrow phase run over chunks of the pair list; the
         // chunk bounds the pairs of all batch elements together (the item buffer holds B
         // chunks), so the workspace does not grow with the batch
-        int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, kmax, B);
+        int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, kmax, B, item_bytes(q.narrow));
         const bool ccd = q.narrow == Narrow::CCD_COLLECT || q.narrow == Narrow::CCD_IMPULSE;
         T *tv = ccd ? ws.get_as<T>(B * n * 18) : nullptr;
         if (ws.failed()) return CCC_ERR_NO_MEMORY;
@@ -2075,7 +2077,7 @@ static size_t resident_bytes(const Query &q) {
     tot += r(sizeof(float) * B * (int64_t)(gT / kFrontier) * 6);
     const int64_t capK = q.cap_candidates > 0 ? q.cap_candidates : 1;
     if (!q.pairs_ext) tot += r(sizeof(int2) * B * capK);
-    const int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, capK, B);
+    const int64_t chunk = pair_chunk_of(q.ctx, q.chunk_ext, capK, B, item_bytes(q.narrow));
     tot += r(esz * B * n * 18) + r(item_bytes(q.narrow) * B * 15 * chunk);
     return tot;
 }

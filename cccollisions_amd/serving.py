@@ -49,7 +49,7 @@ def footprint_bytes(num_triangles: int, max_candidates: int, itemsize: int = 8) 
     n = num_triangles
     resident = (3 * (n // 2 + 1) * 3 + 4 * 9 * n + 3 * n) * itemsize + (2 * n - 1) * (6 * itemsize + 32) + 36 * n
     pairs = max_candidates * n * 8
-    workspace = n * (300 + 18 * itemsize) + 15 * min(max_candidates * n, 1 << 25) * 8
+    workspace = n * (300 + 18 * itemsize) + 15 * min(max_candidates * n, 1 << 26) * 4
     return resident + pairs + workspace
 
 

@@ -81,8 +81,9 @@ int ccc_abi_version(void);
 /* Pairs of candidate triangles per narrow-phase chunk, over all batch elements of a call (no
  * counterpart in the reference, whose narrow phase materialises every candidate's
  * tetrahedra at once, src/collisions_continuous.cu:887-1005).  The sub-primitive items of
- * one chunk (at most 15 per pair, 8 bytes each) are the largest workspace buffer; 0 restores
- * the default of 2^25 pairs (4 GiB of items).  Results do not depend on it. */
+ * one chunk (at most 15 per pair, 4 bytes each in the continuous queries, 8 in the static
+ * ones) are the largest workspace buffer; 0 restores the default of 2^26 pairs (4 GiB of
+ * continuous items), at most 2^28.  Results do not depend on it. */
 ccc_status ccc_context_set_pair_chunk(ccc_context *ctx, int64_t pairs);
 
 /* Morton codes of the trees ccc_bvh / ccc_bvh_motion build (no counterpart in the reference,
@@ -287,7 +288,7 @@ typedef struct {
     int32_t graph;              /* capture each step as one HIP graph (default 1)               */
     int32_t morton;             /* CCC_MORTON_* of the step's trees (one-mesh simulations)      */
     int32_t ccd_blocks_per_cu;  /* grid cap of the narrow-phase kernel (default 128)             */
-    int64_t pair_chunk;         /* candidate pairs per narrow-phase chunk (0: 2^25)            */
+    int64_t pair_chunk;         /* candidate pairs per narrow-phase chunk (0: 2^26)            */
 } ccc_sim_opts;
 void ccc_sim_default_opts(ccc_sim_opts *opts);
 /* x0, x1: device [V, 3]; faces: device [F, 3] int64; masses: device [F, 3]; all copied into

@@ -129,10 +129,12 @@ def test_chunked_narrow_phase_batched(ccc, oracle, small_chunks):
         _check_close(_np(g[1])[i], odx, f"dx {i}")
 
 
-def test_folded_cloth_full_size_first_pass(ccc, oracle):
+def test_folded_cloth_full_size_first_pass(ccc, oracle, small_chunks):
     """The benchmark's 256x256 folded cloth (130,050 triangles): one impulse pass on the GPU
-    (default chunk size: the ~59 M candidate pairs run as two chunks of 2^25) against the
-    oracle's multithreaded pass: K, M, V, R equal, cnt exact, dv / dx within 1e-9."""
+    (the ~59 M candidate pairs run as two chunks of 2^25; the default chunk of 2^26 holds
+    them in one) against the oracle's multithreaded pass: K, M, V, R equal, cnt exact, dv / dx
+    within 1e-9."""
+    small_chunks(1 << 25)
     from cccollisions_amd import scenes
     sc = scenes.folded_cloth(256, seed=2)
     t0, t1 = sc.soups()
