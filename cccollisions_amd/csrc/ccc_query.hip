@@ -28,7 +28,9 @@ constexpr int kTravThreads = 64;   // one wave per traversal block (128: 20.2 ms
 constexpr int kStackShort = 24;
 constexpr int kStackDeep = 32;
 constexpr int kStage = 512;       // staged pairs per wave before a flush
-constexpr int kFrontierDepth = 7; // queries start from the subtrees this deep below the root
+// queries start from the subtrees this deep below the root (k_traverse per pass on the 130k
+// cloth: depth 6 2.85 ms, 7 2.02 ms, 8 2.60 ms)
+constexpr int kFrontierDepth = 7;
 constexpr int kFrontier = 1 << kFrontierDepth;
 
 // Workgroups are dispatched to the eight XCDs round-robin by workgroup ID, and each XCD has
