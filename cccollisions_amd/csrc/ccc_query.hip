@@ -1168,7 +1168,8 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 // comes from a ballot) and the LDS exchanges between phases need only wave-level ordering
 // (wave_sync), not a workgroup barrier, whose fence would also wait for the wave's
 // outstanding global loads and atomics.
-//   A  item -> tetrahedron -> coefficients -> cubic_prepare, in registers;
+//   A  item ((pair index within the chunk) << 4 | type, its candidate pair read one tile
+//      ahead) -> tetrahedron -> coefficients -> cubic_prepare, in registers;
 //   B  the lane's roots inside [0, 1] (at most three) go to an LDS root list (item, root),
 //      placed by a wave-wide scan of the counts;
 //   C  filterCollisions per root in whole rounds of kCcdThreads roots (the tetrahedron is
@@ -1177,7 +1178,8 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 //      scattered through an LDS hash table that reuses the root-list memory: one lane per
 //      collision computes its impulse parameters (gathering the tetrahedron once for the
 //      weights and once for the rest, so that this step needs no more registers than the
-//      CCD phases), then adds its four corners' shares to the table.  The scatter stays in this
+//      CCD phases); lanes l and l ^ 32 exchange them, and each half of the chunk adds its
+//      corners' shares to the table and drains it.  The scatter stays in this
 //      kernel: run as a kernel of its own over per-workgroup collision lists it took 14 ms
 //      per pass on the 130k cloth against ~6 here, where other workgroups' fp64 work hides
 //      its atomic latency.
@@ -1186,8 +1188,8 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 // Each lane's item of the next tile is loaded while this tile is solved.  The cubic's
 // branches diverge within the wave whatever the order of the items, so the items are not
 // grouped by branch.  Occupancy: the impulse mode four waves per SIMD (128 VGPRs).  At five
-// waves the impulse mode spilled about a hundred registers and ran 40% slower; at three 13%
-// slower.  Staging each wave's query triangle in LDS (so that half the corner reads leave
+// waves the impulse mode spilled (once about a hundred registers and 40% slower; with the
+// current code 32 per lane and 40% slower again); at three 13% slower.  Staging each wave's query triangle in LDS (so that half the corner reads leave
 // the vector memory path) made the impulse mode spill and took 196-209 ms per step
 // instead of 177.
 constexpr int kCollectFlush = 64;
