@@ -1204,8 +1204,10 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Waves per SIMD asked of the compiler: the deterministic table's LDS (12.3 KB) allows only
+// 3.25 waves, and at four its registers spilled (37.28 against 37.06 ms per pass at three).
 template <typename T, int MODE, bool DET>
-__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? 4 : 5) void k_ccd(const uint32_t *__restrict__ items_all, int64_t capM,
+__global__ __launch_bounds__(kCcdThreads, MODE == MODE_IMPULSE ? (DET ? 3 : 4) : 5) void k_ccd(const uint32_t *__restrict__ items_all, int64_t capM,
                                                   const int2 *__restrict__ pairs_all, int64_t capK, int64_t c0,
                                                   const T *__restrict__ tv_all,
                                                   const T *__restrict__ masses_all, int64_t n,
