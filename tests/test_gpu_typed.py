@@ -8,8 +8,9 @@ narrow phase:
   * the narrow phase split into many pair chunks (a small chunk size on a small cloth, in
     impulse and collect mode, single and batched);
   * the 256x256 folded cloth (130,050 triangles, the benchmark's workload) at full size:
-    one impulse pass against the oracle's multithreaded pass, with the default chunk size
-    (two chunks of 2^25 pairs).
+    one impulse pass in two chunks of 2^25 pairs against the oracle's multithreaded pass,
+    and the collect mode in one default chunk, its rows turned into impulses and compared
+    with the same pass.
 All calls go through the torch extension, which calls the C ABI."""
 import numpy as np
 import pytest
@@ -129,24 +130,52 @@ def test_chunked_narrow_phase_batched(ccc, oracle, small_chunks):
         _check_close(_np(g[1])[i], odx, f"dx {i}")
 
 
-def test_folded_cloth_full_size_first_pass(ccc, oracle, small_chunks):
-    """The benchmark's 256x256 folded cloth (130,050 triangles): one impulse pass on the GPU
-    (the ~59 M candidate pairs run as two chunks of 2^25; the default chunk of 2^26 holds
-    them in one) against the oracle's multithreaded pass: K, M, V, R equal, cnt exact, dv / dx
-    within 1e-9."""
-    small_chunks(1 << 25)
+@pytest.fixture(scope="module")
+def full_cloth(oracle):
+    """The benchmark's 256x256 folded cloth (130,050 triangles) and the oracle's
+    multithreaded first impulse pass over it: (t0, t1, masses, dv, dx, cnt, counts)."""
     from cccollisions_amd import scenes
     sc = scenes.folded_cloth(256, seed=2)
     t0, t1 = sc.soups()
     m = sc.corner_masses()
     dv, dx, cnt, counts = oracle.impulse_pass(t0, t1, m, nthreads=16)
     assert counts["candidates"] > (1 << 25)
-    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
-    g = ccc.compute_impulses(bb, tree, _t(t0[None]), _t(t1[None]), _t(m[None]), 512, 16)
-    K, M, V, R = ccc.last_counts()[0]
-    assert (K, M, V, R) == (counts["candidates"], counts["subprims"], counts["roots"], counts["collisions"])
+    return t0, t1, m, dv, dx, cnt, counts
+
+
+def _check_pass(g, dv, dx, cnt):
     np.testing.assert_array_equal(_np(g[2])[0], cnt)
     for a, b, name in ((_np(g[0])[0], dv, "dv"), (_np(g[1])[0], dx, "dx")):
         np.testing.assert_array_equal(np.isfinite(a), np.isfinite(b))
         fin = np.isfinite(b)
         _check_close(np.where(fin, a, 0.0), np.where(fin, b, 0.0), name)
+
+
+def test_folded_cloth_full_size_first_pass(ccc, full_cloth, small_chunks):
+    """The 130k cloth: one impulse pass on the GPU (the ~59 M candidate pairs run as two
+    chunks of 2^25; the default chunk of 2^26 holds them in one) against the oracle's pass:
+    K, M, V, R equal, cnt exact, dv / dx within 1e-9."""
+    t0, t1, m, dv, dx, cnt, counts = full_cloth
+    small_chunks(1 << 25)
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    g = ccc.compute_impulses(bb, tree, _t(t0[None]), _t(t1[None]), _t(m[None]), 512, 16)
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, V, R) == (counts["candidates"], counts["subprims"], counts["roots"], counts["collisions"])
+    _check_pass(g, dv, dx, cnt)
+
+
+def test_folded_cloth_full_size_collect(ccc, full_cloth):
+    """The 130k cloth through the collect mode at the default chunk size: K, M, V, R equal
+    the oracle's, and the 34 M collision rows and roots it returns, turned into impulses by
+    compute_impulses_from_collisions, give the oracle's pass (cnt exact, dv / dx within
+    1e-9): the rows and roots are the oracle's collisions."""
+    t0, t1, m, dv, dx, cnt, counts = full_cloth
+    bb, tree = ccc.bvh_motion(_t(t0[None]), _t(t1[None]))
+    mcol = counts["collisions"] // len(t0) + 1
+    gc, gr = ccc.find_collisions_continuous(bb, tree, _t(t0[None]), _t(t1[None]), 512, mcol)
+    K, M, V, R = ccc.last_counts()[0]
+    assert (K, M, V, R) == (counts["candidates"], counts["subprims"], counts["roots"], counts["collisions"])
+    assert int((_np(gc)[0][:, 0] >= 0).sum()) == R
+    del bb, tree
+    g = ccc.compute_impulses_from_collisions(gc, gr, _t(t0[None]), _t(t1[None]), _t(m[None]))
+    _check_pass(g, dv, dx, cnt)
