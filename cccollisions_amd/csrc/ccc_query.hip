@@ -1189,9 +1189,9 @@ template <typename P> __device__ __forceinline__ P *opaque_ptr(P *p) {
 // branches diverge within the wave whatever the order of the items, so the items are not
 // grouped by branch.  Occupancy: the impulse mode four waves per SIMD (128 VGPRs).  At five
 // waves the impulse mode spilled (once about a hundred registers and 40% slower; with the
-// current code 32 per lane and 40% slower again); at three 13% slower.  Staging each wave's query triangle in LDS (so that half the corner reads leave
-// the vector memory path) made the impulse mode spill and took 196-209 ms per step
-// instead of 177.
+// current code 32 per lane and 40% slower again); at three 13% slower.  Staging each wave's
+// query triangle in LDS (so that half the corner reads leave the vector memory path) made
+// the impulse mode spill and took 196-209 ms per step instead of 177.
 constexpr int kCollectFlush = 64;
 
 // Orders the wave's LDS accesses before this point against those after it.  The lanes of a
