@@ -67,6 +67,8 @@ def parse():
                     help="offered load of the open-loop run as a fraction of the saturation rate")
     ap.add_argument("--no-morton", dest="morton", action="store_false",
                     help="skip the Morton-tree comparison on a face-shuffled cloth")
+    ap.add_argument("--no-proximity", dest="proximity", action="store_false",
+                    help="skip the static proximity query on the 7,938-triangle cloth")
     ap.add_argument("--main-only", action="store_true",
                     help="time only the 130k-triangle cloth (no other section)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -75,6 +77,7 @@ def parse():
     args = ap.parse_args()
     if args.main_only:
         args.garment, args.collect, args.batched_sims, args.serving_sims, args.morton = False, False, 0, 0, False
+        args.proximity = False
     return args
 
 
@@ -172,6 +175,29 @@ def collect_section(ccc, torch, sc, args):
     counts = ccc.last_counts()[0]
     return {"calls": 3, "ms_per_call": 1e3 * float(np.median(ts)), "ccd_collect_ms_per_call": kt["ccd_collect"][0] / 3,
             "rows_capacity": sc.num_triangles * args.collect_rows, "counts": list(map(int, counts))}
+
+
+def proximity_section(ccc, torch, scenes):
+    """Static proximity (find_proximity, threshold 2e-3) on a 64x64-vertex static cloth of
+    7,938 triangles: bvh + query per call, inputs resident, median of 20 calls."""
+    v, f = scenes.proximity_cloth(64, seed=1)
+    tri = torch.from_numpy(np.ascontiguousarray(v[f][None])).cuda()
+
+    def call():
+        bb, tree = ccc.bvh(tri)
+        return ccc.find_proximity(bb, tree, tri, 2e-3, 16)
+
+    call()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(20):
+        a = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - a)
+    counts = ccc.last_counts()[0]
+    return {"triangles": int(f.shape[0]), "calls": 20, "ms_per_call": 1e3 * float(np.median(ts)),
+            "p99_ms": 1e3 * float(np.percentile(ts, 99)), "counts": list(map(int, counts))}
 
 
 def batched_section(ccc, torch, scenes, args):
@@ -456,6 +482,8 @@ def main():
         out["garment"] = garment_section(ccc, torch, scenes, args)
     if args.morton:
         out["morton_shuffled"] = morton_section(ccc, torch, scenes, args)
+    if args.proximity:
+        out["proximity"] = proximity_section(ccc, torch, scenes)
     if args.batched_sims:
         out["batched"] = batched_section(ccc, torch, scenes, args)
     if args.serving_sims:
